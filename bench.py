@@ -1,0 +1,205 @@
+"""Benchmark of the in-place 3D R2C + C2R round trip on one MI355X.
+
+Workload (default): 512^3 FP32, input uniform[-1, 1) generated on the device
+from a fixed seed, p = 1.  One step = GPU_FFT_R2C followed by GPU_FFT_C2R on
+the same in-place array (six axis passes).  The inverse carries the fused
+1/(Nx Ny Nz) scale (gfft_plan_set_scale), the normalisation the reference
+harness applies between the two calls (GPU_FFT_TEST_SCRIPT/test.cu:149), so
+repeated steps keep the data at the input's magnitude; the scale is one
+multiply inside the z pass's epilogue and adds no memory traffic.
+
+Prints one JSON line (stdout).  Throughput is GFLOP/s with the 5 N log2 N
+convention per transform (2 transforms per step).  The roofline object is for
+the dominant kernel (the slowest of the six passes): algorithmic bytes per
+launch = 2 S (the half-spectrum S = Nx*Ny*(Nz/2+1)*2*sizeof(real) is read
+once and written once), divided by that pass's mean duration measured with
+HIP events on the exec stream during the timed steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "512^3 FP32 R2C+C2R GFLOP/s & % HBM roofline; mixed-batch p50/p99 latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(n: int, reps: int, workers: int) -> dict:
+    """The FP64 oracle (pocketfft via scipy.fft, multithreaded) on the same
+    transform: R2C + C2R of an n^3 seeded field, median of ``reps`` runs."""
+    import numpy as np
+    from oracle import gfft_oracle as O
+    f = O.pad_real(O.random_field(n, n, n, seed=0))
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        s = O.r2c_p1(f, n, workers=workers)
+        O.c2r_p1(s, n, workers=workers)
+        times.append(time.perf_counter() - t0)
+        del s
+    t = statistics.median(times)
+    flops = 2 * 5.0 * n ** 3 * math.log2(n ** 3)
+    return {
+        "value": round(flops / t / 1e9, 3),
+        "unit": "GFLOP/s",
+        "cores": workers,
+        "kind": "port",
+        "sample": (f"FP64 oracle (scipy.fft pocketfft, {workers} threads) R2C+C2R of {n}^3, median of {reps} "
+                   f"runs ({t * 1e3:.1f} ms each), CPU {cpu_model()}, {os.cpu_count()} logical CPUs"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=512, help="cube edge")
+    ap.add_argument("--prec", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the transform path is single-GPU (p = 1); "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import gpu_fft_amd as G
+
+    n = args.n
+    nx = ny = nz = n
+    K = nz // 2 + 1
+    real_bytes = 4 if args.prec == "f32" else 8
+    S = nx * ny * K * 2 * real_bytes
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    dt = torch.float32 if args.prec == "f32" else torch.float64
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    buf = torch.rand((nx, ny, 2 * K), dtype=dt, device=dev, generator=gen) * 2 - 1
+    buf[:, :, nz:] = 0
+    stream = torch.cuda.current_stream()
+
+    plan = G.Plan(nx, ny, nz, precision=args.prec)
+    plan.set_scale(1.0, 1.0 / (nx * ny * nz))
+    desc = plan.describe()
+    log("plan", json.dumps(desc))
+
+    passes = [(0, False), (1, False), (2, False), (2, True), (1, True), (0, True)]
+    names = ["z_r2c", "y_fwd", "x_fwd", "x_inv", "y_inv", "z_c2r"]
+
+    def step(ev=None):
+        for j, (ax, inv) in enumerate(passes):
+            if ev is not None:
+                ev[j][0].record(stream)
+            plan.axis(ax, inv, buf, stream)
+            if ev is not None:
+                ev[j][1].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region: exactly K steps, per-pass events on the exec stream
+    evs = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)] for _ in passes]
+           for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    t_start.record(stream)
+    for k in range(args.steps):
+        step(evs[k])
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms_total = t_start.elapsed_time(t_end)
+    ms_step = ms_total / args.steps
+
+    per_pass = {}
+    for j, nm in enumerate(names):
+        ts = [evs[k][j][0].elapsed_time(evs[k][j][1]) for k in range(args.steps)]
+        mean = sum(ts) / len(ts)
+        per_pass[nm] = {"ms_mean": round(mean, 5), "ms_min": round(min(ts), 5),
+                        "GBps": round(2 * S / (mean * 1e-3) / 1e9, 1)}
+    dom = max(per_pass, key=lambda k: per_pass[k]["ms_mean"])
+    dom_ms = per_pass[dom]["ms_mean"]
+    achieved = 2 * S / (dom_ms * 1e-3) / 1e9
+
+    # sanity: the array must still hold the (normalised) input's magnitude
+    rms = float(buf[:, :, :nz].float().pow(2).mean().sqrt())
+    finite = bool(torch.isfinite(buf[:, :, :nz]).all())
+    log(f"ms/step {ms_step:.4f}  wall/step {wall / args.steps * 1e3:.4f}  rms {rms:.4f} finite {finite}")
+
+    flops_step = 2 * 5.0 * (nx * ny * nz) * math.log2(nx * ny * nz)
+    gflops = flops_step / (ms_step * 1e-3) / 1e9
+    total_alg = 12 * S
+    out = {
+        "metric": METRIC,
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.prec == "f32" else "f64",
+        "data": "synthetic uniform[-1,1) generated on device (seed 0), pads zero",
+        "config": {"workload": f"{n}^3 {args.prec.upper()} in-place R2C+C2R round trip, p=1",
+                   "nx": nx, "ny": ny, "nz": nz, "procs": 1, "precision": args.prec,
+                   "inverse_scale": "1/(Nx*Ny*Nz) fused"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": 2 * S},
+        "roofline_round_trip": {"alg_bytes": total_alg, "model": "12*S (3 passes x read+write per direction)",
+                                "achieved_GBps": round(total_alg / (ms_step * 1e-3) / 1e9, 1),
+                                "frac": round(total_alg / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                                "compulsory_frac": round(4 * S / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
+        "per_pass": per_pass,
+        "wall_ms_per_step": round(wall / args.steps * 1e3, 5),
+        "output_rms": round(rms, 5),
+        "output_finite": finite,
+    }
+    plan.close()
+    del buf
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        workers = max(1, min(16, os.cpu_count() or 1))
+        log(f"cpu baseline: {workers} threads")
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_reps, workers)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

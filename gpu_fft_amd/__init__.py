@@ -1,0 +1,220 @@
+"""gpu_fft_amd -- MI355X-native in-place 3D R2C/C2R FFT engine.
+
+The compute path is ``libgpufft.so`` (hand-written HIP kernels for gfx950,
+built in-tree from ``gpu_fft_amd/csrc``), reached through its C ABI
+(``include/gpufft/gpufft.h``) with ctypes.  PyTorch only provides device
+memory and streams.  There is no CPU fallback: if the library is missing or
+fails to load, importing the transform API raises.
+
+The reference exposes C++ templates (GPU_FFT/src/GPU_FFT/GPU_FFT.h:114-121);
+``Plan`` mirrors ``INIT_GPU_FFT`` (constructor), ``GPU_FFT_R2C`` (``r2c``) and
+``GPU_FFT_C2R`` (``c2r``) with the same layouts: the real view is
+(Nx/p, Ny, 2K) with K = Nz//2 + 1, the complex view (Nx, Ny/p, K), in place.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+__all__ = [
+    "GfftError", "Plan", "lib", "lib_path", "query_plan", "kdim", "alloc_inplace",
+    "real_view", "complex_view", "F32", "F64",
+]
+
+F32 = 0
+F64 = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def lib_path() -> str:
+    return os.path.join(_HERE, "libgpufft.so")
+
+
+class GfftError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        msg = _strerror(status)
+        super().__init__(f"{what}: gfft status {status} ({msg})" if what else f"gfft status {status} ({msg})")
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libgpufft.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run __graft_entry__.build() or "
+                          f"make -C gpu_fft_amd/csrc")
+    L = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    i64, vp, sz = ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t
+    L.gfft_version.restype = ctypes.c_int
+    L.gfft_strerror.restype = ctypes.c_char_p
+    L.gfft_strerror.argtypes = [ctypes.c_int]
+    L.gfft_plan_create.restype = ctypes.c_int
+    L.gfft_plan_create.argtypes = [ctypes.POINTER(vp), i64, i64, i64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.gfft_plan_destroy.restype = ctypes.c_int
+    L.gfft_plan_destroy.argtypes = [vp]
+    for name in ("gfft_exec_r2c", "gfft_exec_c2r"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [vp, vp, vp]
+    L.gfft_plan_set_scale.restype = ctypes.c_int
+    L.gfft_plan_set_scale.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    L.gfft_exec_axis.restype = ctypes.c_int
+    L.gfft_exec_axis.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
+    for name in ("gfft_plan_device_bytes", "gfft_plan_data_bytes"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [vp, ctypes.POINTER(sz)]
+    L.gfft_plan_describe.restype = ctypes.c_int
+    L.gfft_plan_describe.argtypes = [vp, ctypes.c_char_p, sz]
+    L.gfft_query_plan.restype = ctypes.c_int
+    L.gfft_query_plan.argtypes = [i64, i64, i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, sz]
+    _lib = L
+    return L
+
+
+def _strerror(status: int) -> str:
+    try:
+        return lib().gfft_strerror(status).decode()
+    except Exception:  # pragma: no cover - only when the library is absent
+        return "?"
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise GfftError(status, what)
+
+
+def _prec(precision) -> int:
+    if precision in (F32, "f32", "float32", "float"):
+        return F32
+    if precision in (F64, "f64", "float64", "double"):
+        return F64
+    raise ValueError(f"unknown precision {precision!r}")
+
+
+def kdim(nz: int) -> int:
+    """Half-spectrum length along z, K = Nz//2 + 1."""
+    return nz // 2 + 1
+
+
+def query_plan(nx, ny, nz, procs=1, rank=0, precision="f32") -> dict:
+    """Host-only planning (no GPU needed): the plan JSON, or GfftError."""
+    buf = ctypes.create_string_buffer(8192)
+    _check(lib().gfft_query_plan(nx, ny, nz, procs, rank, _prec(precision), buf, len(buf)), "gfft_query_plan")
+    return json.loads(buf.value.decode())
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _ptr(x) -> ctypes.c_void_p:
+    if isinstance(x, int):
+        return ctypes.c_void_p(x)
+    return ctypes.c_void_p(x.data_ptr())
+
+
+class Plan:
+    """A transform plan on the current HIP device (mirror of INIT_GPU_FFT).
+
+    ``r2c``/``c2r`` take a torch tensor (any view of the in-place array) or a
+    raw device address and run on ``stream`` (default: torch's current
+    stream), asynchronously.
+    """
+
+    def __init__(self, nx: int, ny: int, nz: int, procs: int = 1, rank: int = 0, precision="f32"):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.procs, self.rank = int(procs), int(rank)
+        self.prec = _prec(precision)
+        self.K = kdim(self.nz)
+        h = ctypes.c_void_p()
+        _check(lib().gfft_plan_create(ctypes.byref(h), self.nx, self.ny, self.nz, self.procs, self.rank, self.prec),
+               "gfft_plan_create")
+        self._h = h
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise ValueError("plan destroyed")
+        return self._h
+
+    def r2c(self, data, stream=None) -> None:
+        _check(lib().gfft_exec_r2c(self.handle, _ptr(data), _stream_handle(stream)), "gfft_exec_r2c")
+
+    def c2r(self, data, stream=None) -> None:
+        _check(lib().gfft_exec_c2r(self.handle, _ptr(data), _stream_handle(stream)), "gfft_exec_c2r")
+
+    def set_scale(self, r2c_scale: float = 1.0, c2r_scale: float = 1.0) -> None:
+        """Output scales fused into the last pass of each direction."""
+        _check(lib().gfft_plan_set_scale(self.handle, float(r2c_scale), float(c2r_scale)), "gfft_plan_set_scale")
+
+    def axis(self, axis: int, inverse: bool, data, stream=None) -> None:
+        """One in-place axis pass (0 = z, 1 = y, 2 = x)."""
+        _check(lib().gfft_exec_axis(self.handle, int(axis), int(bool(inverse)), _ptr(data), _stream_handle(stream)),
+               "gfft_exec_axis")
+
+    def describe(self) -> dict:
+        buf = ctypes.create_string_buffer(8192)
+        _check(lib().gfft_plan_describe(self.handle, buf, len(buf)), "gfft_plan_describe")
+        return json.loads(buf.value.decode())
+
+    @property
+    def device_bytes(self) -> int:
+        v = ctypes.c_size_t()
+        _check(lib().gfft_plan_device_bytes(self.handle, ctypes.byref(v)), "gfft_plan_device_bytes")
+        return v.value
+
+    @property
+    def data_bytes(self) -> int:
+        v = ctypes.c_size_t()
+        _check(lib().gfft_plan_data_bytes(self.handle, ctypes.byref(v)), "gfft_plan_data_bytes")
+        return v.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib().gfft_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def alloc_inplace(nx, ny, nz, precision="f32", procs=1, device="cuda"):
+    """A zeroed torch tensor holding one in-place array: real view
+    (Nx/p, Ny, 2K) of float32/float64."""
+    import torch
+    dt = torch.float32 if _prec(precision) == F32 else torch.float64
+    return torch.zeros((nx // procs, ny, 2 * kdim(nz)), dtype=dt, device=device)
+
+
+def real_view(buf, nz):
+    """Unpadded real view (..., Nz) of an in-place buffer (no copy)."""
+    return buf[..., :nz]
+
+
+def complex_view(buf, nx_out, ny_out, nz):
+    """Complex (Nx, Ny/p, K) view of an in-place buffer (no copy)."""
+    import torch
+    return torch.view_as_complex(buf.view(-1, 2)).view(nx_out, ny_out, kdim(nz))

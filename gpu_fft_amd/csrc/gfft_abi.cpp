@@ -1,0 +1,129 @@
+// extern "C" entry points declared in include/gpufft/gpufft.h.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "gfft_plan.h"
+
+struct gfft_plan_s {
+  gfft::PlanCore core;
+  int device = 0;
+};
+
+namespace {
+
+int copy_text(const std::string& s, char* buf, size_t len) {
+  if (!buf) return GFFT_ERR_INVALID_ARG;
+  if (s.size() + 1 > len) return GFFT_ERR_BUFFER_TOO_SMALL;
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return GFFT_OK;
+}
+
+int check_device(const gfft_plan_s* p) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return GFFT_ERR_HIP;
+  return dev == p->device ? GFFT_OK : GFFT_ERR_DEVICE_MISMATCH;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gfft_version(void) { return GFFT_VERSION; }
+
+const char* gfft_strerror(int status) {
+  switch (status) {
+    case GFFT_OK: return "success";
+    case GFFT_ERR_INVALID_ARG: return "invalid argument (null handle or pointer, or bad precision flag)";
+    case GFFT_ERR_INVALID_DIMS: return "invalid dimensions (an edge < 1, procs does not divide Nx and Ny, or overflow)";
+    case GFFT_ERR_UNSUPPORTED_SIZE: return "unsupported edge length (prime factor > 5 or above the LDS-resident limit)";
+    case GFFT_ERR_UNSUPPORTED_PROCS: return "full transform with procs > 1 needs an inter-rank exchange (not provided)";
+    case GFFT_ERR_INVALID_RANK: return "rank outside [0, procs)";
+    case GFFT_ERR_OUT_OF_MEMORY: return "device allocation failed";
+    case GFFT_ERR_HIP: return "HIP runtime or kernel launch error";
+    case GFFT_ERR_DEVICE_MISMATCH: return "current HIP device differs from the plan's device";
+    case GFFT_ERR_BUFFER_TOO_SMALL: return "text buffer too small";
+    default: return "unknown status";
+  }
+}
+
+int gfft_query_plan(int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int prec, char* buf, size_t len) {
+  gfft::PlanCore c;
+  const int st = gfft::plan_host(c, nx, ny, nz, procs, rank, prec);
+  if (st != GFFT_OK) return st;
+  return copy_text(gfft::plan_json(c), buf, len);
+}
+
+int gfft_plan_create(gfft_plan* plan, int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int prec) {
+  if (!plan) return GFFT_ERR_INVALID_ARG;
+  *plan = nullptr;
+  gfft_plan_s* p = new (std::nothrow) gfft_plan_s;
+  if (!p) return GFFT_ERR_OUT_OF_MEMORY;
+  int st = gfft::plan_host(p->core, nx, ny, nz, procs, rank, prec);
+  if (st == GFFT_OK && hipGetDevice(&p->device) != hipSuccess) st = GFFT_ERR_HIP;
+  if (st == GFFT_OK) st = gfft::upload_twiddles(p->core);
+  if (st != GFFT_OK) {
+    gfft::free_twiddles(p->core);
+    delete p;
+    return st;
+  }
+  *plan = p;
+  return GFFT_OK;
+}
+
+int gfft_plan_destroy(gfft_plan plan) {
+  if (!plan) return GFFT_ERR_INVALID_ARG;
+  gfft::free_twiddles(plan->core);
+  delete plan;
+  return GFFT_OK;
+}
+
+int gfft_exec_r2c(gfft_plan plan, void* data, gfft_stream_t stream) {
+  if (!plan || !data) return GFFT_ERR_INVALID_ARG;
+  if (plan->core.procs != 1) return GFFT_ERR_UNSUPPORTED_PROCS;
+  const int st = check_device(plan);
+  if (st != GFFT_OK) return st;
+  return gfft::exec_r2c(plan->core, data, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gfft_exec_c2r(gfft_plan plan, void* data, gfft_stream_t stream) {
+  if (!plan || !data) return GFFT_ERR_INVALID_ARG;
+  if (plan->core.procs != 1) return GFFT_ERR_UNSUPPORTED_PROCS;
+  const int st = check_device(plan);
+  if (st != GFFT_OK) return st;
+  return gfft::exec_c2r(plan->core, data, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gfft_exec_axis(gfft_plan plan, int axis, int inverse, void* data, gfft_stream_t stream) {
+  if (!plan || !data || axis < 0 || axis > 2) return GFFT_ERR_INVALID_ARG;
+  const int st = check_device(plan);
+  if (st != GFFT_OK) return st;
+  return gfft::exec_axis(plan->core, axis, inverse != 0, data, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gfft_plan_set_scale(gfft_plan plan, double r2c_scale, double c2r_scale) {
+  if (!plan) return GFFT_ERR_INVALID_ARG;
+  plan->core.r2c_scale = r2c_scale;
+  plan->core.c2r_scale = c2r_scale;
+  return GFFT_OK;
+}
+
+int gfft_plan_device_bytes(gfft_plan plan, size_t* bytes) {
+  if (!plan || !bytes) return GFFT_ERR_INVALID_ARG;
+  *bytes = plan->core.z.tw_bytes + plan->core.y.tw_bytes + plan->core.x.tw_bytes;
+  return GFFT_OK;
+}
+
+int gfft_plan_data_bytes(gfft_plan plan, size_t* bytes) {
+  if (!plan || !bytes) return GFFT_ERR_INVALID_ARG;
+  const gfft::PlanCore& c = plan->core;
+  *bytes = (size_t)(c.nx / c.procs) * (size_t)c.ny * (size_t)c.K * (c.prec == gfft::F32 ? 8u : 16u);
+  return GFFT_OK;
+}
+
+int gfft_plan_describe(gfft_plan plan, char* buf, size_t len) {
+  if (!plan) return GFFT_ERR_INVALID_ARG;
+  return copy_text(gfft::plan_json(plan->core), buf, len);
+}
+
+}  // extern "C"

@@ -1,0 +1,283 @@
+// Device-side building blocks of the gfx950 FFT engine.
+//
+//  * cx<T>            : POD complex, 8/16-byte aligned (one dwordx2 / dwordx4)
+//  * dft<R, INV>      : radix-2/3/4/5/8 butterflies, FMA form, in registers
+//  * Sched<R...>      : a compile-time mixed-radix Stockham schedule
+//  * RegEngine        : register-resident Stockham; each thread owns P points,
+//                       stages exchange through LDS (one barrier pair each)
+//  * RtSched/gen_*    : the same Stockham recurrence with a run-time schedule,
+//                       LDS ping-pong, for sizes without a compiled schedule
+//
+// Stockham recurrence (all stages, both engines), for radix R and p = product
+// of the radices already applied, butterfly i in [0, N/R), k = i mod p:
+//    a_r = x[i + r N/R] * W_{pR}^{r k}        (r = 0..R-1, W_L = e^{-2 pi i/L})
+//    A   = DFT_R(a)
+//    y[(i - k) R + k + r p] = A_r
+// The output of the last stage is in natural order.  Inverse transforms use
+// the conjugate twiddles and butterflies (sign +1), unnormalised.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gfft {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T a, T b) {
+  cx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+
+// a * w (INV = false) or a * conj(w) (INV = true)
+template <bool INV, typename T>
+__device__ __forceinline__ cx<T> twmul(cx<T> a, cx<T> w) {
+  if (INV) return mk<T>(fma(a.x, w.x, a.y * w.y), fma(a.y, w.x, -(a.x * w.y)));
+  return mk<T>(fma(a.x, w.x, -(a.y * w.y)), fma(a.x, w.y, a.y * w.x));
+}
+// multiply by W_4 of the transform direction: -i forward, +i inverse
+template <bool INV, typename T>
+__device__ __forceinline__ cx<T> rot4(cx<T> a) {
+  return INV ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
+}
+// multiply by the real scalar s and then by W_4 of the direction
+template <bool INV, typename T>
+__device__ __forceinline__ cx<T> rot4s(cx<T> a, T s) {
+  return INV ? mk<T>(-a.y * s, a.x * s) : mk<T>(a.y * s, -a.x * s);
+}
+
+// ---------------------------------------------------------------------------
+// Butterflies (unnormalised DFT of R points, in place)
+// ---------------------------------------------------------------------------
+template <int R, bool INV, typename T>
+struct Dft;
+
+template <bool INV, typename T>
+struct Dft<2, INV, T> {
+  __device__ __forceinline__ static void run(cx<T>* a) {
+    cx<T> t = a[0] - a[1];
+    a[0] = a[0] + a[1];
+    a[1] = t;
+  }
+};
+
+template <bool INV, typename T>
+struct Dft<3, INV, T> {
+  __device__ __forceinline__ static void run(cx<T>* a) {
+    const T s3 = (T)0.86602540378443864676372317075293618;  // sin(2 pi/3)
+    cx<T> b = a[1] + a[2];
+    cx<T> d = a[1] - a[2];
+    cx<T> y0 = a[0] + b;
+    cx<T> t = mk<T>(fma((T)-0.5, b.x, a[0].x), fma((T)-0.5, b.y, a[0].y));
+    cx<T> u = rot4s<INV, T>(d, s3);  // -i s3 d (forward)
+    a[0] = y0;
+    a[1] = t + u;
+    a[2] = t - u;
+  }
+};
+
+template <bool INV, typename T>
+struct Dft<4, INV, T> {
+  __device__ __forceinline__ static void run(cx<T>* a) {
+    cx<T> t0 = a[0] + a[2], t1 = a[0] - a[2];
+    cx<T> t2 = a[1] + a[3], t3 = rot4<INV>(a[1] - a[3]);
+    a[0] = t0 + t2;
+    a[2] = t0 - t2;
+    a[1] = t1 + t3;
+    a[3] = t1 - t3;
+  }
+};
+
+template <bool INV, typename T>
+struct Dft<5, INV, T> {
+  __device__ __forceinline__ static void run(cx<T>* a) {
+    const T c1 = (T)0.30901699437494742410229341718281906;   // cos(2 pi/5)
+    const T c2 = (T)-0.80901699437494742410229341718281906;  // cos(4 pi/5)
+    const T s1 = (T)0.95105651629515357211643933337938214;   // sin(2 pi/5)
+    const T s2 = (T)0.58778525229247312916870595463907277;   // sin(4 pi/5)
+    cx<T> b1 = a[1] + a[4], d1 = a[1] - a[4];
+    cx<T> b2 = a[2] + a[3], d2 = a[2] - a[3];
+    cx<T> y0 = a[0] + b1 + b2;
+    cx<T> t1 = mk<T>(fma(c2, b2.x, fma(c1, b1.x, a[0].x)), fma(c2, b2.y, fma(c1, b1.y, a[0].y)));
+    cx<T> t2 = mk<T>(fma(c1, b2.x, fma(c2, b1.x, a[0].x)), fma(c1, b2.y, fma(c2, b1.y, a[0].y)));
+    cx<T> u1 = mk<T>(fma(s1, d1.x, s2 * d2.x), fma(s1, d1.y, s2 * d2.y));
+    cx<T> u2 = mk<T>(fma(s2, d1.x, -(s1 * d2.x)), fma(s2, d1.y, -(s1 * d2.y)));
+    u1 = rot4<INV>(u1);  // -i u1 forward
+    u2 = rot4<INV>(u2);
+    a[0] = y0;
+    a[1] = t1 + u1;
+    a[4] = t1 - u1;
+    a[2] = t2 + u2;
+    a[3] = t2 - u2;
+  }
+};
+
+template <bool INV, typename T>
+struct Dft<8, INV, T> {
+  __device__ __forceinline__ static void run(cx<T>* a) {
+    const T h = (T)0.70710678118654752440084436210484904;  // 1/sqrt(2)
+    cx<T> t0 = a[0] + a[4], t1 = a[0] - a[4];
+    cx<T> t2 = a[2] + a[6], t3 = rot4<INV>(a[2] - a[6]);
+    cx<T> t4 = a[1] + a[5], t5 = a[1] - a[5];
+    cx<T> t6 = a[3] + a[7], t7 = rot4<INV>(a[3] - a[7]);
+    cx<T> e0 = t0 + t2, e2 = t0 - t2, e1 = t1 + t3, e3 = t1 - t3;
+    cx<T> o0 = t4 + t6, o2 = t4 - t6, o1 = t5 + t7, o3 = t5 - t7;
+    // W8^1, W8^2, W8^3 of the direction applied to o1, o2, o3
+    cx<T> w1, w3;
+    if (INV) {
+      w1 = mk<T>((o1.x - o1.y) * h, (o1.x + o1.y) * h);
+      w3 = mk<T>(-(o3.x + o3.y) * h, (o3.x - o3.y) * h);
+    } else {
+      w1 = mk<T>((o1.x + o1.y) * h, (o1.y - o1.x) * h);
+      w3 = mk<T>((o3.y - o3.x) * h, -(o3.x + o3.y) * h);
+    }
+    cx<T> w2 = rot4<INV>(o2);
+    a[0] = e0 + o0;
+    a[4] = e0 - o0;
+    a[1] = e1 + w1;
+    a[5] = e1 - w1;
+    a[2] = e2 + w2;
+    a[6] = e2 - w2;
+    a[3] = e3 + w3;
+    a[7] = e3 - w3;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Compile-time schedules
+// ---------------------------------------------------------------------------
+template <int... Rs>
+struct Sched {
+  static constexpr int S = sizeof...(Rs);
+  static constexpr int R[S] = {Rs...};
+  static constexpr int N = (Rs * ...);
+  static constexpr int p(int s) {
+    int v = 1;
+    for (int i = 0; i < s; ++i) v *= R[i];
+    return v;
+  }
+  // offset (in complex entries) of stage s's twiddles in the plan table:
+  // stage s >= 1 stores p(s) * (R[s]-1) entries, k-major.
+  static constexpr int twoff(int s) {
+    int o = 0;
+    for (int i = 1; i < s; ++i) o += p(i) * (R[i] - 1);
+    return o;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Register-resident engine.  Thread t (0 <= t < TPT = N/P) owns butterflies
+// i = t + TPT*q (q < P/R) of every stage; v[q*R + r] is input/output r of
+// butterfly q.  After run(), v[q*R + r] holds X[t + TPT*q + r*N/R_last].
+// ---------------------------------------------------------------------------
+template <typename T, bool INV, typename SC, int P>
+struct RegEngine {
+  using C = cx<T>;
+  static constexpr int N = SC::N;
+  static constexpr int TPT = N / P;
+  static_assert(N % P == 0, "P must divide N");
+
+  template <int s>
+  __device__ __forceinline__ static void bfly(C (&v)[P], int t, const C* __restrict__ tw) {
+    constexpr int R = SC::R[s];
+    constexpr int p = SC::p(s);
+    static_assert(P % R == 0, "every radix must divide P");
+#pragma unroll
+    for (int q = 0; q < P / R; ++q) {
+      if constexpr (p > 1) {
+        const int i = t + TPT * q;
+        const int k = i % p;
+        const C* w = tw + SC::twoff(s) + k * (R - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[q * R + r] = twmul<INV>(v[q * R + r], w[r - 1]);
+      }
+      Dft<R, INV, T>::run(&v[q * R]);
+    }
+  }
+
+  template <int s, typename L>
+  __device__ __forceinline__ static void put(C (&v)[P], int t, L& lds) {
+    constexpr int R = SC::R[s];
+    constexpr int p = SC::p(s);
+#pragma unroll
+    for (int q = 0; q < P / R; ++q) {
+      const int i = t + TPT * q;
+      const int k = i % p;
+      const int j = (i - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds.at(j + r * p) = v[q * R + r];
+    }
+  }
+
+  template <int s, typename L>
+  __device__ __forceinline__ static void get(C (&v)[P], int t, L& lds) {
+    constexpr int R = SC::R[s];
+#pragma unroll
+    for (int q = 0; q < P / R; ++q) {
+      const int i = t + TPT * q;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q * R + r] = lds.at(i + r * (N / R));
+    }
+  }
+
+  template <int s, typename L>
+  __device__ __forceinline__ static void run_from(C (&v)[P], int t, L& lds, const C* __restrict__ tw) {
+    bfly<s>(v, t, tw);
+    if constexpr (s + 1 < SC::S) {
+      __syncthreads();
+      put<s>(v, t, lds);
+      __syncthreads();
+      get<s + 1>(v, t, lds);
+      run_from<s + 1>(v, t, lds, tw);
+    }
+  }
+
+  // v holds stage-0 inputs: v[q*R0 + r] = x[t + TPT*q + r*N/R0]
+  template <typename L>
+  __device__ __forceinline__ static void run(C (&v)[P], int t, L& lds, const C* __restrict__ tw) {
+    run_from<0>(v, t, lds, tw);
+  }
+
+  // element index of stage-0 input / last-stage output slot (q, r)
+  __device__ __forceinline__ static constexpr int in_index(int t, int q, int r) {
+    return t + TPT * q + r * (N / SC::R[0]);
+  }
+  __device__ __forceinline__ static constexpr int out_index(int t, int q, int r) {
+    return t + TPT * q + r * (N / SC::R[SC::S - 1]);
+  }
+  static constexpr int R0 = SC::R[0];
+  static constexpr int RL = SC::R[SC::S - 1];
+};
+
+// ---------------------------------------------------------------------------
+// Run-time schedule (any product of 2, 3, 4, 5, 8)
+// ---------------------------------------------------------------------------
+constexpr int kMaxStages = 16;
+struct RtSched {
+  int N;
+  int S;
+  int R[kMaxStages];
+  int p[kMaxStages];
+  int twoff[kMaxStages];
+};
+
+// XCD-aware block remap: hardware hands consecutive workgroup ids to the 8
+// XCDs round-robin; remap so that each XCD walks a contiguous range of tiles
+// (neighbouring tiles share straddled cache lines in that XCD's L2).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks) {
+  constexpr uint32_t X = 8;
+  const uint32_t xcd = b % X, j = b / X;
+  const uint32_t base = nblocks / X, rem = nblocks % X;
+  return (xcd < rem) ? xcd * (base + 1) + j : rem * (base + 1) + (xcd - rem) * base + j;
+}
+
+}  // namespace gfft

@@ -1,0 +1,42 @@
+// Plan structures shared by the planner (gfft_plan.cpp) and the C ABI
+// (gfft_abi.cpp).  Internal.
+#pragma once
+#include <string>
+
+#include "gfft_internal.h"
+#include "gpufft/gpufft.h"
+
+namespace gfft {
+
+struct AxisPlan {
+  int N = 0;
+  bool rows = false;     // z pass (two-for-one real rows) vs strided columns
+  bool active = false;   // false for a length-1 strided axis (identity)
+  const KernelEntry* reg_fwd = nullptr;  // compiled variants (both or neither)
+  const KernelEntry* reg_inv = nullptr;
+  Schedule sched{};      // radix order used (fixes the twiddle table)
+  RtSchedHost rt{};      // the same order for the generic kernels
+  void* d_tw = nullptr;  // device twiddle table
+  size_t tw_bytes = 0;
+};
+
+struct PlanCore {
+  int64_t nx = 0, ny = 0, nz = 0, K = 0;
+  int procs = 1, rank = 0, prec = 0;
+  double r2c_scale = 1.0, c2r_scale = 1.0;  // fused output scales (1 = unnormalised)
+  AxisPlan z, y, x;
+};
+
+int validate_dims(int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int prec);
+int plan_host(PlanCore& c, int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int prec);
+std::string plan_json(const PlanCore& c);
+int upload_twiddles(PlanCore& c);
+void free_twiddles(PlanCore& c);
+int exec_r2c(const PlanCore& c, void* data, hipStream_t s);
+int exec_c2r(const PlanCore& c, void* data, hipStream_t s);
+int exec_axis(const PlanCore& c, int axis, bool inverse, void* data, hipStream_t s);
+ColGeom y_geom(const PlanCore& c, double scale);
+ColGeom x_geom(const PlanCore& c, double scale);
+RowGeom z_geom(const PlanCore& c, double scale);
+
+}  // namespace gfft

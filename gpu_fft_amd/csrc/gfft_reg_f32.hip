@@ -1,0 +1,28 @@
+// Compiled register-engine variants, FP32.  Column tiles are 16 columns wide
+// (16 x 8 B = 128 B per row segment) except where the tile would not fit LDS.
+#include "gfft_schedules.h"
+
+namespace gfft {
+
+int registry_f32(const KernelEntry** out) {
+  static const KernelEntry table[] = {
+      //        type   schedule  P_rows G  P_cols CW
+      GFFT_SIZE(float, Sc32, 8, 64, 8, 16),
+      GFFT_SIZE(float, Sc48, 12, 64, 12, 16),
+      GFFT_SIZE(float, Sc64, 8, 32, 8, 16),
+      GFFT_SIZE(float, Sc96, 24, 32, 24, 16),
+      GFFT_SIZE(float, Sc128, 8, 16, 8, 16),
+      GFFT_SIZE(float, Sc160, 40, 32, 40, 16),
+      GFFT_SIZE(float, Sc192, 24, 16, 24, 16),
+      GFFT_SIZE(float, Sc256, 8, 8, 8, 16),
+      GFFT_SIZE(float, Sc384, 24, 8, 24, 16),
+      GFFT_SIZE(float, Sc512, 8, 4, 8, 16),
+      GFFT_SIZE(float, Sc768, 24, 4, 24, 16),
+      GFFT_SIZE(float, Sc1024, 8, 2, 16, 16),
+      GFFT_SIZE(float, Sc2048, 8, 1, 16, 8),
+  };
+  *out = table;
+  return (int)(sizeof(table) / sizeof(table[0]));
+}
+
+}  // namespace gfft

@@ -1,0 +1,123 @@
+/*
+ * gpufft.h -- C ABI of the MI355X (gfx950) 3D R2C/C2R FFT engine.
+ *
+ * This is the drop-in boundary that replaces the reference's hot path:
+ *   INIT_GPU_FFT<T1,T2>(Nx,Ny,Nz,procs,rank,comm)
+ *        GPU_FFT/src/GPU_FFT/GPU_FFT.h:114-115, GPU_FFT_init.cu:41-93
+ *   GPU_FFT_R2C<T1,T2>(T1*)   GPU_FFT.h:117-118, FFT_R2C.cu:41-70
+ *   GPU_FFT_C2R<T1,T2>(T2*)   GPU_FFT.h:120-121, FFT_C2R.cu:41-70
+ * The reference exposes only C++ templates over process-global state
+ * (GPU_FFT.cu:43-87); this ABI is handle-based, stream-ordered and returns
+ * status codes.  include/GPU_FFT/GPU_FFT.h rebuilds the reference templates on
+ * top of it.
+ *
+ * Data layout (identical to the reference at p = 1, Readme.md:47-52):
+ *   real input   (Nx/p, Ny, 2K) reals, row-major, K = Nz/2 + 1 (integer
+ *                division), the last 2K - Nz reals of every row are padding;
+ *   complex out  (Nx, Ny/p, K) complex (interleaved re, im), kz fastest,
+ *                in the same bytes (in place).
+ * Forward uses exp(-2 pi i jk/N); neither direction normalises, so
+ * C2R(R2C(f)) = Nx*Ny*Nz*f.  C2R overwrites its input; its pad slots are
+ * unspecified.  C2R ignores the imaginary parts of the kz = 0 and (even Nz)
+ * kz = Nz/2 bins after the x and y passes (numpy irfftn convention).
+ *
+ * All entry points are thread-safe for distinct plans.  Execution is
+ * asynchronous on the given HIP stream (NULL = the null stream), exactly like
+ * the reference, which returns with the last FFT still in flight
+ * (FFT_R2C.cu:69, caller syncs at GPU_FFT_TEST_SCRIPT/test.cu:146).
+ */
+#ifndef GPUFFT_GPUFFT_H_
+#define GPUFFT_GPUFFT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFFT_VERSION 100 /* 1.0.0 */
+
+/* Compatible with hipStream_t (which is `struct ihipStream_t*`). */
+typedef struct ihipStream_t* gfft_stream_t;
+
+typedef struct gfft_plan_s* gfft_plan;
+
+typedef enum {
+  GFFT_F32 = 0, /* float  / float2   (reference T1_f / T2_f, transitions.h:77-78) */
+  GFFT_F64 = 1  /* double / double2  (reference T1_d / T2_d, transitions.h:79-80) */
+} gfft_prec;
+
+typedef enum {
+  GFFT_OK = 0,
+  GFFT_ERR_INVALID_ARG = 1,       /* NULL handle/pointer, bad precision flag */
+  GFFT_ERR_INVALID_DIMS = 2,      /* a dimension < 1, p does not divide Nx and Ny, sizes overflow */
+  GFFT_ERR_UNSUPPORTED_SIZE = 3,  /* an edge has a prime factor > 5 or exceeds the LDS-resident limit */
+  GFFT_ERR_UNSUPPORTED_PROCS = 4, /* full transform requested with procs > 1 (needs an exchange) */
+  GFFT_ERR_INVALID_RANK = 5,      /* rank outside [0, procs) */
+  GFFT_ERR_OUT_OF_MEMORY = 6,     /* device allocation failed */
+  GFFT_ERR_HIP = 7,               /* a HIP runtime call or kernel launch failed */
+  GFFT_ERR_DEVICE_MISMATCH = 8,   /* executed with another current device than the plan's */
+  GFFT_ERR_BUFFER_TOO_SMALL = 9   /* text query buffer too small */
+} gfft_status;
+
+/* Version number (GFFT_VERSION of the built library). */
+int gfft_version(void);
+
+/* Static description of a status code. */
+const char* gfft_strerror(int status);
+
+/* Plan a transform of an Nx x Ny x Nz grid on the current HIP device.
+ * Replaces INIT_GPU_FFT (GPU_FFT_init.cu:41-93): validates the geometry,
+ * picks a kernel per axis, uploads exact twiddle tables (computed in extended
+ * precision on the host, rounded once).  Unlike the reference it allocates
+ * no work buffer: every pass is in place.  procs > 1 plans the local share of
+ * a slab decomposition (see gfft_exec_r2c_local). */
+int gfft_plan_create(gfft_plan* plan, int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int prec);
+
+/* Release a plan and its device tables (the reference never frees). */
+int gfft_plan_destroy(gfft_plan plan);
+
+/* In-place forward 3D R2C (replaces GPU_FFT_R2C, FFT_R2C.cu:41-70). */
+int gfft_exec_r2c(gfft_plan plan, void* data, gfft_stream_t stream);
+
+/* In-place inverse 3D C2R, unnormalised (replaces GPU_FFT_C2R,
+ * FFT_C2R.cu:41-70). */
+int gfft_exec_c2r(gfft_plan plan, void* data, gfft_stream_t stream);
+
+/* Run one axis pass of the plan in place (axis 0 = z rows, 1 = y, 2 = x;
+ * inverse 0 = the R2C direction, 1 = the C2R direction), for per-pass
+ * measurement and for callers that interleave their own work between passes.
+ * R2C = passes (z,0), (y,0), (x,0); C2R = (x,1), (y,1), (z,1).  A length-1
+ * strided axis is a no-op.  No counterpart in the reference (its passes are
+ * hipFFT plans, GPU_FFT_init.cu:86-91). */
+int gfft_exec_axis(gfft_plan plan, int axis, int inverse, void* data, gfft_stream_t stream);
+
+/* Output scale factors fused into the last pass of each direction (default
+ * 1.0 and 1.0: unnormalised, the reference's semantics).  c2r_scale =
+ * 1/(Nx*Ny*Nz) gives the normalised inverse that the reference harness
+ * obtains with a separate kernel (GPU_FFT_TEST_SCRIPT/test.cu:53-65,149) at
+ * no extra memory traffic.  Not thread-safe against concurrent execs of the
+ * same plan. */
+int gfft_plan_set_scale(gfft_plan plan, double r2c_scale, double c2r_scale);
+
+/* Device bytes held by the plan (twiddle tables), for admission control. */
+int gfft_plan_device_bytes(gfft_plan plan, size_t* bytes);
+
+/* Bytes of the caller's in-place array for this plan's local share:
+ * (Nx/p) * Ny * K complex values. */
+int gfft_plan_data_bytes(gfft_plan plan, size_t* bytes);
+
+/* Text (JSON) description of the passes the plan will launch. */
+int gfft_plan_describe(gfft_plan plan, char* buf, size_t len);
+
+/* Host-only planning of a geometry (no device work): same validation and
+ * kernel choice as gfft_plan_create, written as JSON into buf.  Returns the
+ * status gfft_plan_create would return for a geometry problem. */
+int gfft_query_plan(int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int prec, char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUFFT_GPUFFT_H_ */

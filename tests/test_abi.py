@@ -1,0 +1,104 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/gpufft/gpufft.h declares, and its host-side planner validates
+geometries and picks kernels as documented."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import gpu_fft_amd as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gfft_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = G.lib()
+    names = declared_functions("gpufft/gpufft.h")
+    assert "gfft_exec_r2c" in names and "gfft_plan_create" in names
+    for n in names:
+        assert hasattr(L, n), n
+    assert L.gfft_version() == 100
+
+
+def test_strerror_covers_codes():
+    for code in range(0, 10):
+        s = G.lib().gfft_strerror(code).decode()
+        assert s and s != "unknown status"
+    assert G.lib().gfft_strerror(99).decode() == "unknown status"
+
+
+@pytest.mark.parametrize("args,status", [
+    ((0, 8, 8, 1, 0, 0), 2),     # edge < 1
+    ((8, 8, -4, 1, 0, 0), 2),
+    ((8, 8, 8, 0, 0, 0), 2),     # procs < 1
+    ((8, 8, 8, 1, 1, 0), 5),     # rank outside [0, procs)
+    ((8, 8, 8, 2, -1, 0), 5),
+    ((12, 8, 8, 8, 0, 0), 2),    # procs does not divide Nx
+    ((8, 12, 8, 8, 0, 0), 2),    # ... nor Ny
+    ((8, 8, 8, 1, 0, 7), 1),     # bad precision flag
+    ((8, 8, 14, 1, 0, 0), 3),    # prime factor 7
+    ((11, 8, 8, 1, 0, 1), 3),    # prime factor 11
+    ((8, 8, 8, 1, 0, 0), 0),
+    ((768, 768, 768, 1, 0, 0), 0),
+    ((128, 1024, 1024, 8, 3, 0), 0),
+])
+def test_query_plan_validation(args, status):
+    buf = ctypes.create_string_buffer(8192)
+    assert G.lib().gfft_query_plan(*args, buf, len(buf)) == status
+
+
+def test_query_plan_buffer_too_small():
+    buf = ctypes.create_string_buffer(8)
+    assert G.lib().gfft_query_plan(8, 8, 8, 1, 0, 0, buf, len(buf)) == 9
+
+
+def test_null_arguments_rejected_without_gpu():
+    L = G.lib()
+    assert L.gfft_plan_create(None, 8, 8, 8, 1, 0, 0) == 1
+    assert L.gfft_plan_destroy(None) == 1
+    assert L.gfft_exec_r2c(None, None, None) == 1
+    assert L.gfft_exec_c2r(None, None, None) == 1
+    v = ctypes.c_size_t()
+    assert L.gfft_plan_device_bytes(None, ctypes.byref(v)) == 1
+
+
+def test_plan_choice_512_uses_register_kernels():
+    d = G.query_plan(512, 512, 512, precision="f32")
+    assert d["K"] == 257
+    for ax in ("z", "y", "x"):
+        assert d[ax]["engine"] == "register" and d[ax]["radices"] == [8, 8, 8]
+    assert d["z"]["pass"] == "rows_two_for_one" and d["x"]["pass"] == "strided_columns"
+
+
+def test_plan_choice_generic_and_identity():
+    d = G.query_plan(1, 75, 243, precision="f64")
+    assert d["x"]["pass"] == "identity"
+    assert d["y"]["engine"] == "generic" and sorted(d["y"]["radices"]) == [3, 5, 5]
+    assert d["z"]["engine"] == "generic" and d["z"]["radices"] == [3, 3, 3, 3, 3]
+
+
+def test_twiddle_table_sizes():
+    # stage s >= 1 stores p_s (R_s - 1) complex entries: 512 = 8*8*8 -> 8*7 + 64*7
+    d = G.query_plan(512, 512, 512, precision="f32")
+    assert d["z"]["twiddle_bytes"] == (8 * 7 + 64 * 7) * 8
+    d = G.query_plan(512, 512, 512, precision="f64")
+    assert d["z"]["twiddle_bytes"] == (8 * 7 + 64 * 7) * 16
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 25, 27, 45, 75, 81, 100, 125, 135,
+                               225, 243, 250, 375, 405, 480, 500, 640, 720, 1000])
+def test_all_235_smooth_sizes_plan(n):
+    d = G.query_plan(n, n, n, precision="f32")
+    for ax in ("z", "y", "x"):
+        r = d[ax]["radices"]
+        prod = 1
+        for v in r:
+            prod *= v
+        assert prod == n and set(r) <= {2, 3, 4, 5, 8}
