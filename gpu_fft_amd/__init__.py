@@ -29,7 +29,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def lib_path() -> str:
-    return os.path.join(_HERE, "libgpufft.so")
+    """The in-tree library; GPU_FFT_AMD_LIB may point at another build of
+    the same sources (developer A/B comparisons)."""
+    return os.environ.get("GPU_FFT_AMD_LIB") or os.path.join(_HERE, "libgpufft.so")
 
 
 class GfftError(RuntimeError):
@@ -64,6 +66,12 @@ def lib() -> ctypes.CDLL:
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [vp, vp, vp]
+    L.gfft_plan_tune.restype = ctypes.c_int
+    L.gfft_plan_tune.argtypes = [vp, vp, vp, ctypes.c_int]
+    L.gfft_plan_set_variant.restype = ctypes.c_int
+    L.gfft_plan_set_variant.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.gfft_plan_set_chunking.restype = ctypes.c_int
+    L.gfft_plan_set_chunking.argtypes = [vp, ctypes.c_int64]
     L.gfft_plan_set_scale.restype = ctypes.c_int
     L.gfft_plan_set_scale.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.gfft_exec_axis.restype = ctypes.c_int
@@ -156,6 +164,19 @@ class Plan:
 
     def c2r(self, data, stream=None) -> None:
         _check(lib().gfft_exec_c2r(self.handle, _ptr(data), _stream_handle(stream)), "gfft_exec_c2r")
+
+    def tune(self, scratch, stream=None, reps: int = 5) -> dict:
+        """Autotune every axis on ``scratch`` (contents destroyed); returns
+        the plan description with the chosen variants."""
+        _check(lib().gfft_plan_tune(self.handle, _ptr(scratch), _stream_handle(stream), int(reps)), "gfft_plan_tune")
+        return self.describe()
+
+    def set_variant(self, axis: int, variant: int) -> None:
+        _check(lib().gfft_plan_set_variant(self.handle, int(axis), int(variant)), "gfft_plan_set_variant")
+
+    def set_chunking(self, planes: int) -> None:
+        """Alternate z and y passes per block of ``planes`` x-planes (0 = off)."""
+        _check(lib().gfft_plan_set_chunking(self.handle, int(planes)), "gfft_plan_set_chunking")
 
     def set_scale(self, r2c_scale: float = 1.0, c2r_scale: float = 1.0) -> None:
         """Output scales fused into the last pass of each direction."""

@@ -101,6 +101,24 @@ int gfft_exec_axis(gfft_plan plan, int axis, int inverse, void* data, gfft_strea
   return gfft::exec_axis(plan->core, axis, inverse != 0, data, reinterpret_cast<hipStream_t>(stream));
 }
 
+int gfft_plan_set_variant(gfft_plan plan, int axis, int variant) {
+  if (!plan) return GFFT_ERR_INVALID_ARG;
+  return gfft::set_variant(plan->core, axis, variant);
+}
+
+int gfft_plan_tune(gfft_plan plan, void* scratch, gfft_stream_t stream, int reps) {
+  if (!plan || !scratch) return GFFT_ERR_INVALID_ARG;
+  const int st = check_device(plan);
+  if (st != GFFT_OK) return st;
+  return gfft::tune(plan->core, scratch, reinterpret_cast<hipStream_t>(stream), reps);
+}
+
+int gfft_plan_set_chunking(gfft_plan plan, int64_t planes) {
+  if (!plan || planes < 0) return GFFT_ERR_INVALID_ARG;
+  plan->core.chunk_planes = planes;
+  return GFFT_OK;
+}
+
 int gfft_plan_set_scale(gfft_plan plan, double r2c_scale, double c2r_scale) {
   if (!plan) return GFFT_ERR_INVALID_ARG;
   plan->core.r2c_scale = r2c_scale;

@@ -44,6 +44,61 @@ __device__ __forceinline__ cx<T> twmul(cx<T> a, cx<T> w) {
   if (INV) return mk<T>(fma(a.x, w.x, a.y * w.y), fma(a.y, w.x, -(a.x * w.y)));
   return mk<T>(fma(a.x, w.x, -(a.y * w.y)), fma(a.x, w.y, a.y * w.x));
 }
+// Global accesses of the passes.  GFFT_NONTEMPORAL=1 adds the nontemporal
+// hint; measured on MI355X it halves the strided passes (row segments that
+// straddle cache lines are then fetched once per tile instead of once per
+// L2), so the default build uses plain accesses.
+#ifndef GFFT_NONTEMPORAL
+#define GFFT_NONTEMPORAL 0
+#endif
+template <typename T>
+struct VecOf;
+template <>
+struct VecOf<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct VecOf<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+#if GFFT_NONTEMPORAL
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ cx<T> ld_stream(const cx<T>* p) {
+#if GFFT_NONTEMPORAL
+  const typename VecOf<T>::type v = __builtin_nontemporal_load(reinterpret_cast<const typename VecOf<T>::type*>(p));
+  return mk<T>(v.x, v.y);
+#else
+  return *p;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+#if GFFT_NONTEMPORAL
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(cx<T>* p, cx<T> v) {
+#if GFFT_NONTEMPORAL
+  typename VecOf<T>::type w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<typename VecOf<T>::type*>(p));
+#else
+  *p = v;
+#endif
+}
+
 // multiply by W_4 of the transform direction: -i forward, +i inverse
 template <bool INV, typename T>
 __device__ __forceinline__ cx<T> rot4(cx<T> a) {
@@ -174,12 +229,34 @@ struct Sched {
   }
 };
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations (lgkmcnt(0)), not for its outstanding global loads, so register
+// prefetches of the next tile stay in flight across the stage exchanges.
+// The empty asm statements keep the compiler from moving memory accesses
+// across the barrier.  s_waitcnt immediate (gfx9 encoding): vmcnt = 63,
+// expcnt = 7, lgkmcnt = 0.
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_s_barrier();
+  __asm__ volatile("" ::: "memory");
+}
+
+template <bool LB>
+__device__ __forceinline__ void stage_barrier() {
+  if constexpr (LB)
+    lds_barrier();
+  else
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // Register-resident engine.  Thread t (0 <= t < TPT = N/P) owns butterflies
 // i = t + TPT*q (q < P/R) of every stage; v[q*R + r] is input/output r of
 // butterfly q.  After run(), v[q*R + r] holds X[t + TPT*q + r*N/R_last].
+// LB selects LDS-only barriers between stages (see lds_barrier).
 // ---------------------------------------------------------------------------
-template <typename T, bool INV, typename SC, int P>
+template <typename T, bool INV, typename SC, int P, bool LB = false>
 struct RegEngine {
   using C = cx<T>;
   static constexpr int N = SC::N;
@@ -233,9 +310,9 @@ struct RegEngine {
   __device__ __forceinline__ static void run_from(C (&v)[P], int t, L& lds, const C* __restrict__ tw) {
     bfly<s>(v, t, tw);
     if constexpr (s + 1 < SC::S) {
-      __syncthreads();
+      stage_barrier<LB>();
       put<s>(v, t, lds);
-      __syncthreads();
+      stage_barrier<LB>();
       get<s + 1>(v, t, lds);
       run_from<s + 1>(v, t, lds, tw);
     }

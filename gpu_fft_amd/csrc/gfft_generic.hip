@@ -208,7 +208,9 @@ int gen_pairs_per_block(int N) {
 
 template <typename T>
 size_t rows_lds_bytes(int N, int G) {
-  return (size_t)G * (size_t)(2 * N + 2) * sizeof(cx<T>);
+  // two buffers of G pairs, each pair (N + 2) entries: the C2R kernel stages
+  // 2K <= N + 2 half-spectrum bins per pair in the second buffer
+  return (size_t)2 * G * (size_t)(N + 2) * sizeof(cx<T>);
 }
 
 template <typename T>

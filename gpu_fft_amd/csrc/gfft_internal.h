@@ -61,6 +61,8 @@ struct KernelEntry {
   int tile;         // pairs (rows) or columns (cols) per workgroup
   int threads;      // threads per workgroup
   void* fn;         // RowsLaunchFn or ColsLaunchFn
+  int persist;      // 0: one tile per workgroup; k: persistent, k workgroups per CU
+  int order;        // persistent tile order: 0 contiguous ranges, 1 round robin
 };
 
 // Registry of compiled variants; returns the number of entries.

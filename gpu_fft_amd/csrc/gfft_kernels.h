@@ -43,68 +43,90 @@ struct ColLds {
 // ---------------------------------------------------------------------------
 // z-axis R2C, register engine
 // ---------------------------------------------------------------------------
-template <typename T, typename SC, int P, int G>
+template <typename T, typename SC, int P, int G, bool LB = false>
 struct RowsR2C {
-  using E = RegEngine<T, false, SC, P>;
+  using E = RegEngine<T, false, SC, P, LB>;
   using C = cx<T>;
   static constexpr int N = SC::N;
   static constexpr int TPT = E::TPT;
   static constexpr int NT = G * TPT;
-  static constexpr int LROW = RowPad<N>::lrow(N / 2 + 1);
+  static constexpr int KC = N / 2 + 1;  // half-spectrum bins per row (= RowGeom::K)
+  static constexpr int LROW = RowPad<N>::lrow(KC);
 
-  __device__ __forceinline__ static void tile(T* __restrict__ data, RowGeom g, const C* __restrict__ tw,
-                                              int64_t pair0, C* smem) {
-    const int gi = threadIdx.x / TPT, t = threadIdx.x % TPT;
-    const int K = g.K;
-    const int64_t ra = 2 * (pair0 + gi), rb = ra + 1;
+  // stage-0 inputs of pair `pair` (rows 2*pair and 2*pair + 1): z = a + i b
+  __device__ __forceinline__ static void load(C (&v)[P], const T* __restrict__ data, const RowGeom& g, int64_t pair,
+                                              int t) {
+    const int64_t ra = 2 * pair, rb = ra + 1;
     const bool va = ra < g.nrows, vb = rb < g.nrows;
-    const T* A = data + ra * 2 * (int64_t)K;
-    const T* B = A + 2 * (int64_t)K;
-    C v[P];
+    const T* A = data + ra * 2 * (int64_t)KC;
+    const T* B = A + 2 * (int64_t)KC;
 #pragma unroll
     for (int q = 0; q < P / E::R0; ++q)
 #pragma unroll
       for (int r = 0; r < E::R0; ++r) {
         const int n = E::in_index(t, q, r);
-        v[q * E::R0 + r] = mk<T>(va ? A[n] : (T)0, vb ? B[n] : (T)0);
+        v[q * E::R0 + r] = mk<T>(va ? ld_stream(A + n) : (T)0, vb ? ld_stream(B + n) : (T)0);
       }
-    RowLds<T, N> L{smem + gi * LROW};
-    E::run(v, t, L, tw);
-    __syncthreads();
+  }
+
+  // after E::run: Z goes through LDS, is split into the two half spectra and
+  // stored over the pair's own rows
+  __device__ __forceinline__ static void finish(C (&v)[P], T* __restrict__ data, const RowGeom& g, int64_t pair,
+                                                int t, RowLds<T, N>& L) {
+    stage_barrier<LB>();
 #pragma unroll
     for (int q = 0; q < P / E::RL; ++q)
 #pragma unroll
       for (int r = 0; r < E::RL; ++r) L.at(E::out_index(t, q, r)) = v[q * E::RL + r];
-    __syncthreads();
-    C* Ac = reinterpret_cast<C*>(data) + ra * K;
-    C* Bc = Ac + K;
+    stage_barrier<LB>();
+    const int64_t ra = 2 * pair, rb = ra + 1;
+    const bool va = ra < g.nrows, vb = rb < g.nrows;
+    C* Ac = reinterpret_cast<C*>(data) + ra * KC;
+    C* Bc = Ac + KC;
     const T h = (T)(0.5 * g.scale);
-    for (int k = t; k < K; k += TPT) {
-      const C zk = L.at(k), zm = L.at(k == 0 ? 0 : N - k);
-      const C xa = mk<T>((zk.x + zm.x) * h, (zk.y - zm.y) * h);
-      const C xb = mk<T>((zk.y + zm.y) * h, (zm.x - zk.x) * h);
-      if (va) Ac[k] = xa;
-      if (vb) Bc[k] = xb;
+    constexpr int NI = (KC + TPT - 1) / TPT;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int k = t + i * TPT;
+      if ((NI * TPT == KC) || k < KC) {
+        const C zk = L.at(k), zm = L.at(k == 0 ? 0 : N - k);
+        const C xa = mk<T>((zk.x + zm.x) * h, (zk.y - zm.y) * h);
+        const C xb = mk<T>((zk.y + zm.y) * h, (zm.x - zk.x) * h);
+        if (va) st_stream(Ac + k, xa);
+        if (vb) st_stream(Bc + k, xb);
+      }
     }
+  }
+
+  __device__ __forceinline__ static void tile(T* __restrict__ data, RowGeom g, const C* __restrict__ tw,
+                                              int64_t pair0, C* smem) {
+    const int gi = threadIdx.x / TPT, t = threadIdx.x % TPT;
+    RowLds<T, N> L{smem + gi * LROW};
+    C v[P];
+    load(v, data, g, pair0 + gi, t);
+    E::run(v, t, L, tw);
+    finish(v, data, g, pair0 + gi, t, L);
   }
 };
 
 // ---------------------------------------------------------------------------
 // z-axis C2R, register engine
 // ---------------------------------------------------------------------------
-template <typename T, typename SC, int P, int G>
+template <typename T, typename SC, int P, int G, bool LB = false>
 struct RowsC2R {
-  using E = RegEngine<T, true, SC, P>;
+  using E = RegEngine<T, true, SC, P, LB>;
   using C = cx<T>;
   static constexpr int N = SC::N;
   static constexpr int TPT = E::TPT;
   static constexpr int NT = G * TPT;
-  static constexpr int LROW = RowPad<N>::lrow(N / 2 + 1);
+  static constexpr int KC = N / 2 + 1;
+  static constexpr int NI = (KC + TPT - 1) / TPT;  // half-spectrum bins per thread and row
+  static constexpr int LROW = RowPad<N>::lrow(KC);
 
-  __device__ __forceinline__ static C hermitian_z(const C* S, int K, int n) {
+  __device__ __forceinline__ static C hermitian_z(const C* S, int n) {
     const bool lo = 2 * n <= N;
     const int m = lo ? n : N - n;
-    C xa = S[m], xb = S[K + m];
+    C xa = S[m], xb = S[KC + m];
     if (n == 0 || 2 * n == N) {
       xa.y = (T)0;
       xb.y = (T)0;
@@ -113,32 +135,47 @@ struct RowsC2R {
     return lo ? mk<T>(xa.x - xb.y, xa.y + xb.x) : mk<T>(xa.x + xb.y, xb.x - xa.y);
   }
 
-  __device__ __forceinline__ static void tile(T* __restrict__ data, RowGeom g, const C* __restrict__ tw,
-                                              int64_t pair0, C* smem) {
-    const int gi = threadIdx.x / TPT, t = threadIdx.x % TPT;
-    const int K = g.K;
-    const int64_t ra = 2 * (pair0 + gi), rb = ra + 1;
+  // both half spectra of the pair (all loads issued before any use)
+  __device__ __forceinline__ static void load(C (&xa)[NI], C (&xb)[NI], const T* __restrict__ data,
+                                              const RowGeom& g, int64_t pair, int t) {
+    const int64_t ra = 2 * pair, rb = ra + 1;
     const bool va = ra < g.nrows, vb = rb < g.nrows;
-    const C* Ac = reinterpret_cast<const C*>(data) + ra * K;
-    const C* Bc = Ac + K;
-    C* S = smem + gi * LROW;
-    for (int k = t; k < K; k += TPT) {
-      S[k] = va ? Ac[k] : mk<T>(0, 0);
-      S[K + k] = vb ? Bc[k] : mk<T>(0, 0);
+    const C* Ac = reinterpret_cast<const C*>(data) + ra * KC;
+    const C* Bc = Ac + KC;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int k = t + i * TPT;
+      const bool ok = (NI * TPT == KC) || k < KC;
+      xa[i] = (va && ok) ? ld_stream(Ac + k) : mk<T>(0, 0);
+      xb[i] = (vb && ok) ? ld_stream(Bc + k) : mk<T>(0, 0);
     }
-    __syncthreads();
-    C v[P];
+  }
+
+  // LDS staging of the spectra, then the stage-0 inputs of the complex FFT
+  __device__ __forceinline__ static void build(C (&v)[P], const C (&xa)[NI], const C (&xb)[NI], C* S, int t) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int k = t + i * TPT;
+      if ((NI * TPT == KC) || k < KC) {
+        S[k] = xa[i];
+        S[KC + k] = xb[i];
+      }
+    }
+    stage_barrier<LB>();
 #pragma unroll
     for (int q = 0; q < P / E::R0; ++q)
 #pragma unroll
-      for (int r = 0; r < E::R0; ++r) v[q * E::R0 + r] = hermitian_z(S, K, E::in_index(t, q, r));
-    RowLds<T, N> L{S};
-    E::run(v, t, L, tw);
-    // every read of this workgroup's rows happened before the staging barrier
-    T* A = data + ra * 2 * (int64_t)K;
-    T* B = A + 2 * (int64_t)K;
-    const T sc = (T)g.scale;
+      for (int r = 0; r < E::R0; ++r) v[q * E::R0 + r] = hermitian_z(S, E::in_index(t, q, r));
+  }
+
+  __device__ __forceinline__ static void store(C (&v)[P], T* __restrict__ data, const RowGeom& g, int64_t pair,
+                                               int t) {
+    const int64_t ra = 2 * pair, rb = ra + 1;
+    const bool va = ra < g.nrows, vb = rb < g.nrows;
+    T* A = data + ra * 2 * (int64_t)KC;
+    T* B = A + 2 * (int64_t)KC;
     if (g.scale != 1.0) {
+      const T sc = (T)g.scale;
 #pragma unroll
       for (int i = 0; i < P; ++i) v[i] = mk<T>(v[i].x * sc, v[i].y * sc);
     }
@@ -148,9 +185,22 @@ struct RowsC2R {
       for (int r = 0; r < E::RL; ++r) {
         const int n = E::out_index(t, q, r);
         const C z = v[q * E::RL + r];
-        if (va) A[n] = z.x;
-        if (vb) B[n] = z.y;
+        if (va) st_stream(A + n, z.x);
+        if (vb) st_stream(B + n, z.y);
       }
+  }
+
+  __device__ __forceinline__ static void tile(T* __restrict__ data, RowGeom g, const C* __restrict__ tw,
+                                              int64_t pair0, C* smem) {
+    const int gi = threadIdx.x / TPT, t = threadIdx.x % TPT;
+    C* S = smem + gi * LROW;
+    C xa[NI], xb[NI], v[P];
+    load(xa, xb, data, g, pair0 + gi, t);
+    build(v, xa, xb, S, t);
+    RowLds<T, N> L{S};
+    E::run(v, t, L, tw);
+    // every read of this workgroup's rows happened before the staging barrier
+    store(v, data, g, pair0 + gi, t);
   }
 };
 
@@ -177,7 +227,7 @@ struct ColsPass {
     for (int q = 0; q < P / E::R0; ++q)
 #pragma unroll
       for (int r = 0; r < E::R0; ++r)
-        v[q * E::R0 + r] = valid ? p[(int64_t)E::in_index(t, q, r) * g.rstride] : mk<T>(0, 0);
+        v[q * E::R0 + r] = valid ? ld_stream(p + (int64_t)E::in_index(t, q, r) * g.rstride) : mk<T>(0, 0);
     ColLds<T, CW> L{smem, c};
     E::run(v, t, L, tw);
     if (g.scale != 1.0) {
@@ -189,7 +239,7 @@ struct ColsPass {
 #pragma unroll
       for (int q = 0; q < P / E::RL; ++q)
 #pragma unroll
-        for (int r = 0; r < E::RL; ++r) p[(int64_t)E::out_index(t, q, r) * g.rstride] = v[q * E::RL + r];
+        for (int r = 0; r < E::RL; ++r) st_stream(p + (int64_t)E::out_index(t, q, r) * g.rstride, v[q * E::RL + r]);
     }
   }
 };
@@ -221,6 +271,162 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPass<T, INV, SC, P,
   K_::tile(data, g, tw, tile, smem);
 }
 
+// Persistent, software-pipelined column pass.  Workgroup w owns the
+// contiguous tile range [w*T/G, (w+1)*T/G) (XCD-aware order, so one XCD walks
+// neighbouring tiles); while tile i runs its LDS stages, the global loads of
+// tile i+1 are already in flight in a second register set.  Stage barriers
+// wait for LDS traffic only, so the prefetch is not drained by them.
+template <typename T, bool INV, typename SC, int P, int CW>
+struct ColsPipe {
+  using E = RegEngine<T, INV, SC, P, true>;
+  using C = cx<T>;
+  static constexpr int N = SC::N;
+  static constexpr int TPT = E::TPT;
+  static constexpr int NT = CW * TPT;
+
+  __device__ __forceinline__ static void load(C (&v)[P], const C* __restrict__ data, const ColGeom& g, uint32_t tile,
+                                              int c, int t) {
+    const uint32_t col = tile * CW + c;
+    const bool valid = col < g.ncols;
+    const int64_t base = valid ? (int64_t)(col / g.inner) * g.outer + (int64_t)(col % g.inner) : 0;
+    const C* p = data + base;
+#pragma unroll
+    for (int q = 0; q < P / E::R0; ++q)
+#pragma unroll
+      for (int r = 0; r < E::R0; ++r)
+        v[q * E::R0 + r] = valid ? ld_stream(p + (int64_t)E::in_index(t, q, r) * g.rstride) : mk<T>(0, 0);
+  }
+
+  __device__ __forceinline__ static void store(C (&v)[P], C* __restrict__ data, const ColGeom& g, uint32_t tile, int c,
+                                               int t) {
+    const uint32_t col = tile * CW + c;
+    if (col >= g.ncols) return;
+    const int64_t base = (int64_t)(col / g.inner) * g.outer + (int64_t)(col % g.inner);
+    C* p = data + base;
+    if (g.scale != 1.0) {
+      const T sc = (T)g.scale;
+#pragma unroll
+      for (int i = 0; i < P; ++i) v[i] = mk<T>(v[i].x * sc, v[i].y * sc);
+    }
+#pragma unroll
+    for (int q = 0; q < P / E::RL; ++q)
+#pragma unroll
+      for (int r = 0; r < E::RL; ++r) st_stream(p + (int64_t)E::out_index(t, q, r) * g.rstride, v[q * E::RL + r]);
+  }
+};
+
+// Tiles of a persistent workgroup: ORD 0 = the contiguous range
+// [w*T/G, (w+1)*T/G); ORD 1 = round robin w, w+G, w+2G, ... (neighbouring
+// tiles run concurrently on neighbouring workgroups of one XCD).
+struct TileWalk {
+  uint32_t first, count, step;
+  __device__ __forceinline__ uint32_t at(uint32_t k) const { return first + k * step; }
+};
+template <int ORD>
+__device__ __forceinline__ TileWalk tile_walk(uint32_t ntiles) {
+  const uint32_t G = gridDim.x;
+  const uint32_t w = xcd_tile(blockIdx.x, G);
+  TileWalk tw;
+  if (ORD == 0) {
+    tw.first = (uint32_t)(((uint64_t)w * ntiles) / G);
+    tw.count = (uint32_t)(((uint64_t)(w + 1) * ntiles) / G) - tw.first;
+    tw.step = 1;
+  } else {
+    tw.first = w;
+    tw.count = w < ntiles ? (ntiles - w + G - 1) / G : 0;
+    tw.step = G;
+  }
+  return tw;
+}
+
+// Twiddles of a persistent kernel live in LDS: a global twiddle load would
+// make the in-order vector-memory counter wait for the prefetched tile too.
+template <typename SC, typename C>
+__device__ __forceinline__ void copy_twiddles(C* stw, const C* __restrict__ tw, int nthreads) {
+  for (int i = threadIdx.x; i < SC::twoff(SC::S); i += nthreads) stw[i] = tw[i];
+}
+template <typename SC>
+constexpr int twiddle_count() {
+  return SC::twoff(SC::S) > 0 ? SC::twoff(SC::S) : 1;
+}
+
+template <typename T, bool INV, typename SC, int P, int CW, int ORD>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPipe<T, INV, SC, P, CW>::NT))) void k_cols_pipe(
+    cx<T>* data, ColGeom g, const cx<T>* tw, uint32_t ntiles) {
+  using K_ = ColsPipe<T, INV, SC, P, CW>;
+  __shared__ cx<T> smem[SC::N * CW];
+  __shared__ cx<T> stw[twiddle_count<SC>()];
+  const TileWalk walk = tile_walk<ORD>(ntiles);
+  if (walk.count == 0) return;  // uniform across the workgroup
+  copy_twiddles<SC>(stw, tw, K_::NT);
+  const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+  ColLds<T, CW> L{smem, c};
+  cx<T> cur[P], nxt[P];
+  K_::load(nxt, data, g, walk.at(0), c, t);
+  for (uint32_t k = 0; k < walk.count; ++k) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) cur[i] = nxt[i];
+    if (k + 1 < walk.count) K_::load(nxt, data, g, walk.at(k + 1), c, t);
+    // the first stage uses no twiddles; two barriers precede the first
+    // twiddle read, so the LDS copy above is complete by then
+    K_::E::run(cur, t, L, stw);
+    K_::store(cur, data, g, walk.at(k), c, t);
+  }
+}
+
+// Persistent, software-pipelined z passes (tile = G row pairs).
+template <typename T, typename SC, int P, int G, int ORD>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, RowsR2C<T, SC, P, G, true>::NT))) void k_rows_r2c_pipe(
+    T* data, RowGeom g, const cx<T>* tw, uint32_t ntiles) {
+  using K_ = RowsR2C<T, SC, P, G, true>;
+  __shared__ cx<T> smem[G * K_::LROW];
+  __shared__ cx<T> stw[twiddle_count<SC>()];
+  const TileWalk walk = tile_walk<ORD>(ntiles);
+  if (walk.count == 0) return;
+  copy_twiddles<SC>(stw, tw, K_::NT);
+  const int gi = threadIdx.x / K_::TPT, t = threadIdx.x % K_::TPT;
+  RowLds<T, SC::N> L{smem + gi * K_::LROW};
+  cx<T> cur[P], nxt[P];
+  K_::load(nxt, data, g, (int64_t)walk.at(0) * G + gi, t);
+  for (uint32_t k = 0; k < walk.count; ++k) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) cur[i] = nxt[i];
+    if (k + 1 < walk.count) K_::load(nxt, data, g, (int64_t)walk.at(k + 1) * G + gi, t);
+    K_::E::run(cur, t, L, stw);
+    K_::finish(cur, data, g, (int64_t)walk.at(k) * G + gi, t, L);
+  }
+}
+
+template <typename T, typename SC, int P, int G, int ORD>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, RowsC2R<T, SC, P, G, true>::NT))) void k_rows_c2r_pipe(
+    T* data, RowGeom g, const cx<T>* tw, uint32_t ntiles) {
+  using K_ = RowsC2R<T, SC, P, G, true>;
+  __shared__ cx<T> smem[G * K_::LROW];
+  __shared__ cx<T> stw[twiddle_count<SC>()];
+  const TileWalk walk = tile_walk<ORD>(ntiles);
+  if (walk.count == 0) return;
+  copy_twiddles<SC>(stw, tw, K_::NT);
+  const int gi = threadIdx.x / K_::TPT, t = threadIdx.x % K_::TPT;
+  cx<T>* S = smem + gi * K_::LROW;
+  RowLds<T, SC::N> L{S};
+  cx<T> ca[K_::NI], cb[K_::NI], na[K_::NI], nb[K_::NI], v[P];
+  K_::load(na, nb, data, g, (int64_t)walk.at(0) * G + gi, t);
+  for (uint32_t k = 0; k < walk.count; ++k) {
+#pragma unroll
+    for (int i = 0; i < K_::NI; ++i) {
+      ca[i] = na[i];
+      cb[i] = nb[i];
+    }
+    if (k + 1 < walk.count) K_::load(na, nb, data, g, (int64_t)walk.at(k + 1) * G + gi, t);
+    lds_barrier();  // the previous tile's stage reads of S are done
+    K_::build(v, ca, cb, S, t);
+    K_::E::run(v, t, L, stw);
+    K_::store(v, data, g, (int64_t)walk.at(k) * G + gi, t);
+  }
+}
+
+int device_cu_count();
+
 // Host launchers with the registry's signatures.
 template <typename T, typename SC, int P, int G>
 hipError_t launch_rows_r2c(void* data, RowGeom g, const void* tw, hipStream_t s) {
@@ -248,6 +454,41 @@ hipError_t launch_cols(void* data, ColGeom g, const void* tw, hipStream_t s) {
   if (blocks <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_cols<T, INV, SC, P, CW>), dim3((uint32_t)blocks), dim3(ColsPass<T, INV, SC, P, CW>::NT), 0,
                      s, static_cast<cx<T>*>(data), g, static_cast<const cx<T>*>(tw));
+  return hipGetLastError();
+}
+
+inline int64_t persistent_blocks(int64_t ntiles, int occ) {
+  int64_t blocks = (int64_t)device_cu_count() * occ;
+  return blocks < ntiles ? blocks : ntiles;
+}
+
+template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD>
+hipError_t launch_cols_pipe(void* data, ColGeom g, const void* tw, hipStream_t s) {
+  const int64_t ntiles = ((int64_t)g.ncols + CW - 1) / CW;
+  if (ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_cols_pipe<T, INV, SC, P, CW, ORD>), dim3((uint32_t)persistent_blocks(ntiles, OCC)),
+                     dim3(ColsPipe<T, INV, SC, P, CW>::NT), 0, s, static_cast<cx<T>*>(data), g,
+                     static_cast<const cx<T>*>(tw), (uint32_t)ntiles);
+  return hipGetLastError();
+}
+
+template <typename T, typename SC, int P, int G, int OCC, int ORD>
+hipError_t launch_rows_r2c_pipe(void* data, RowGeom g, const void* tw, hipStream_t s) {
+  const int64_t ntiles = ((g.nrows + 1) / 2 + G - 1) / G;
+  if (ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_rows_r2c_pipe<T, SC, P, G, ORD>), dim3((uint32_t)persistent_blocks(ntiles, OCC)),
+                     dim3(RowsR2C<T, SC, P, G, true>::NT), 0, s, static_cast<T*>(data), g,
+                     static_cast<const cx<T>*>(tw), (uint32_t)ntiles);
+  return hipGetLastError();
+}
+
+template <typename T, typename SC, int P, int G, int OCC, int ORD>
+hipError_t launch_rows_c2r_pipe(void* data, RowGeom g, const void* tw, hipStream_t s) {
+  const int64_t ntiles = ((g.nrows + 1) / 2 + G - 1) / G;
+  if (ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_rows_c2r_pipe<T, SC, P, G, ORD>), dim3((uint32_t)persistent_blocks(ntiles, OCC)),
+                     dim3(RowsC2R<T, SC, P, G, true>::NT), 0, s, static_cast<T*>(data), g,
+                     static_cast<const cx<T>*>(tw), (uint32_t)ntiles);
   return hipGetLastError();
 }
 
@@ -280,6 +521,48 @@ template <typename T, bool INV, typename SC, int P, int CW>
 KernelEntry entry_cols() {
   return KernelEntry{prec_of<T>(), INV ? COLS_INV : COLS_FWD, SC::N, make_sched<SC>(), P, CW,
                      ColsPass<T, INV, SC, P, CW>::NT, reinterpret_cast<void*>(&launch_cols<T, INV, SC, P, CW>)};
+}
+
+template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD>
+KernelEntry entry_cols_pipe() {
+  return KernelEntry{prec_of<T>(),
+                     INV ? COLS_INV : COLS_FWD,
+                     SC::N,
+                     make_sched<SC>(),
+                     P,
+                     CW,
+                     ColsPipe<T, INV, SC, P, CW>::NT,
+                     reinterpret_cast<void*>(&launch_cols_pipe<T, INV, SC, P, CW, OCC, ORD>),
+                     OCC,
+                     ORD};
+}
+
+template <typename T, typename SC, int P, int G, int OCC, int ORD>
+KernelEntry entry_rows_r2c_pipe() {
+  return KernelEntry{prec_of<T>(),
+                     ROWS_R2C,
+                     SC::N,
+                     make_sched<SC>(),
+                     P,
+                     G,
+                     RowsR2C<T, SC, P, G, true>::NT,
+                     reinterpret_cast<void*>(&launch_rows_r2c_pipe<T, SC, P, G, OCC, ORD>),
+                     OCC,
+                     ORD};
+}
+
+template <typename T, typename SC, int P, int G, int OCC, int ORD>
+KernelEntry entry_rows_c2r_pipe() {
+  return KernelEntry{prec_of<T>(),
+                     ROWS_C2R,
+                     SC::N,
+                     make_sched<SC>(),
+                     P,
+                     G,
+                     RowsC2R<T, SC, P, G, true>::NT,
+                     reinterpret_cast<void*>(&launch_rows_c2r_pipe<T, SC, P, G, OCC, ORD>),
+                     OCC,
+                     ORD};
 }
 
 }  // namespace gfft

@@ -10,9 +10,11 @@
 //   C2R: x columns -> y columns -> z rows
 #include "gfft_plan.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -52,12 +54,33 @@ bool factor_235(int64_t n, std::vector<int>& radices) {
   return true;
 }
 
-const KernelEntry* find_entry(int prec, int kind, int n) {
+int registry(int prec, const KernelEntry** tab) { return prec == F32 ? registry_f32(tab) : registry_f64(tab); }
+
+bool same_sched(const Schedule& a, const Schedule& b) {
+  if (a.N != b.N || a.S != b.S) return false;
+  for (int i = 0; i < a.S; ++i)
+    if (a.R[i] != b.R[i]) return false;
+  return true;
+}
+
+// All compiled (forward, inverse) pairs for a size, default first; only
+// variants with the default's radix order (one twiddle table per axis).
+std::vector<std::pair<const KernelEntry*, const KernelEntry*>> find_pairs(int prec, int kf, int ki, int n) {
+  std::vector<std::pair<const KernelEntry*, const KernelEntry*>> out;
   const KernelEntry* tab = nullptr;
-  const int cnt = prec == F32 ? registry_f32(&tab) : registry_f64(&tab);
-  for (int i = 0; i < cnt; ++i)
-    if (tab[i].kind == kind && tab[i].N == n) return &tab[i];
-  return nullptr;
+  const int cnt = registry(prec, &tab);
+  for (int i = 0; i < cnt; ++i) {
+    if (tab[i].kind != kf || tab[i].N != n) continue;
+    if (!out.empty() && !same_sched(out[0].first->sched, tab[i].sched)) continue;
+    for (int j = 0; j < cnt; ++j)
+      if (tab[j].kind == ki && tab[j].N == n && tab[j].P == tab[i].P && tab[j].tile == tab[i].tile &&
+          tab[j].persist == tab[i].persist && tab[j].order == tab[i].order &&
+          same_sched(tab[j].sched, tab[i].sched)) {
+        out.emplace_back(&tab[i], &tab[j]);
+        break;
+      }
+  }
+  return out;
 }
 
 // Stage twiddles W_{pR}^{rk} = exp(-2 pi i r k / (pR)), stage s >= 1,
@@ -113,19 +136,21 @@ int plan_axis(AxisPlan& a, int prec, int64_t n, bool rows, int64_t K) {
   if (n > (1 << 20) || !factor_235(n, rad)) return GFFT_ERR_UNSUPPORTED_SIZE;
   const int kf = rows ? ROWS_R2C : COLS_FWD;
   const int ki = rows ? ROWS_C2R : COLS_INV;
-  a.reg_fwd = find_entry(prec, kf, (int)n);
-  a.reg_inv = find_entry(prec, ki, (int)n);
-  if (a.reg_fwd && a.reg_inv) {
-    a.sched = a.reg_fwd->sched;
+  a.cands = find_pairs(prec, kf, ki, (int)n);
+  if (!a.cands.empty()) {
+    a.sched = a.cands[0].first->sched;
   } else {
-    a.reg_fwd = a.reg_inv = nullptr;
     if ((int)rad.size() > kMaxRadices) return GFFT_ERR_UNSUPPORTED_SIZE;
     a.sched.N = (int)n;
     a.sched.S = (int)rad.size();
     for (size_t i = 0; i < rad.size(); ++i) a.sched.R[i] = rad[i];
-    const size_t lds = rows ? generic_rows_lds(prec, (int)n, (int)K) : generic_cols_lds(prec, (int)n);
-    if (lds == 0) return GFFT_ERR_UNSUPPORTED_SIZE;
   }
+  const size_t lds = rows ? generic_rows_lds(prec, (int)n, (int)K) : generic_cols_lds(prec, (int)n);
+  if (lds != 0) a.cands.emplace_back(nullptr, nullptr);
+  if (a.cands.empty()) return GFFT_ERR_UNSUPPORTED_SIZE;
+  a.variant = 0;
+  a.reg_fwd = a.cands[0].first;
+  a.reg_inv = a.cands[0].second;
   a.rt = to_rt(a.sched);
   return GFFT_OK;
 }
@@ -148,13 +173,28 @@ std::string axis_json(const char* name, const AxisPlan& a) {
   const KernelEntry* e = a.reg_fwd;
   std::snprintf(buf, sizeof buf,
                 "\"%s\":{\"n\":%d,\"pass\":\"%s\",\"engine\":\"%s\",\"radices\":%s,\"points_per_thread\":%d,"
-                "\"tile\":%d,\"threads\":%d,\"twiddle_bytes\":%zu}",
+                "\"tile\":%d,\"threads\":%d,\"twiddle_bytes\":%zu,\"variant\":%d,\"candidates\":%d,"
+                "\"persist\":%d,\"order\":%d,\"tuned_ms\":%.6f}",
                 name, a.N, a.rows ? "rows_two_for_one" : "strided_columns", e ? "register" : "generic",
-                sched_str(a.sched).c_str(), e ? e->P : 0, e ? e->tile : 0, e ? e->threads : 256, a.tw_bytes);
+                sched_str(a.sched).c_str(), e ? e->P : 0, e ? e->tile : 0, e ? e->threads : 256, a.tw_bytes,
+                a.variant, (int)a.cands.size(), e ? e->persist : 0, e ? e->order : 0, a.tuned_ms);
   return buf;
 }
 
 }  // namespace
+
+// Compute units of the current device (cached per device id).
+int device_cu_count() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cache[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cache[dev] = n;
+  }
+  return cache[dev];
+}
 
 int validate_dims(int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int prec) {
   if (prec != F32 && prec != F64) return GFFT_ERR_INVALID_ARG;
@@ -300,7 +340,132 @@ int exec_axis(const PlanCore& c, int axis, bool inverse, void* data, hipStream_t
   return e == hipSuccess ? GFFT_OK : GFFT_ERR_HIP;
 }
 
+AxisPlan* axis_ptr(PlanCore& c, int axis) {
+  return axis == 0 ? &c.z : axis == 1 ? &c.y : axis == 2 ? &c.x : nullptr;
+}
+
+int set_variant(PlanCore& c, int axis, int variant) {
+  AxisPlan* a = axis_ptr(c, axis);
+  if (!a || !a->active || variant < 0 || variant >= (int)a->cands.size()) return GFFT_ERR_INVALID_ARG;
+  a->variant = variant;
+  a->reg_fwd = a->cands[variant].first;
+  a->reg_inv = a->cands[variant].second;
+  a->tuned_ms = -1;
+  return GFFT_OK;
+}
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune_cache;  // geometry key -> variant index
+
+std::string tune_key(const PlanCore& c, int axis) {
+  char b[256];
+  if (axis == 0) {
+    const RowGeom g = z_geom(c, 1.0);
+    std::snprintf(b, sizeof b, "p%d:rows:%d:%lld", c.prec, c.z.N, (long long)g.nrows);
+  } else {
+    const ColGeom g = axis == 1 ? y_geom(c, 1.0) : x_geom(c, 1.0);
+    std::snprintf(b, sizeof b, "p%d:cols:%d:%u:%u:%lld:%lld", c.prec, axis == 1 ? c.y.N : c.x.N, g.ncols, g.inner,
+                  (long long)g.outer, (long long)g.rstride);
+  }
+  return b;
+}
+}  // namespace
+
+// Time every candidate of every axis (forward + inverse pass on `scratch`,
+// whose contents are destroyed) and keep the fastest.  Results are cached per
+// geometry for the life of the process.
+int tune(PlanCore& c, void* scratch, hipStream_t s, int reps) {
+  if (reps < 1) reps = 1;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return GFFT_ERR_HIP;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return GFFT_ERR_HIP;
+  }
+  int st = GFFT_OK;
+  for (int axis = 0; axis < 3 && st == GFFT_OK; ++axis) {
+    AxisPlan* a = axis_ptr(c, axis);
+    if (!a->active || a->cands.size() < 2) continue;
+    const std::string key = tune_key(c, axis);
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end() && it->second < (int)a->cands.size()) {
+        set_variant(c, axis, it->second);
+        continue;
+      }
+    }
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v < (int)a->cands.size() && st == GFFT_OK; ++v) {
+      set_variant(c, axis, v);
+      // warm-up pair, then `reps` timed pairs
+      if ((st = exec_axis(c, axis, false, scratch, s)) != GFFT_OK) break;
+      if ((st = exec_axis(c, axis, true, scratch, s)) != GFFT_OK) break;
+      if (hipEventRecord(e0, s) != hipSuccess) st = GFFT_ERR_HIP;
+      for (int r = 0; r < reps && st == GFFT_OK; ++r) {
+        st = exec_axis(c, axis, false, scratch, s);
+        if (st == GFFT_OK) st = exec_axis(c, axis, true, scratch, s);
+      }
+      if (st != GFFT_OK) break;
+      float ms = 0;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        st = GFFT_ERR_HIP;
+        break;
+      }
+      ms /= reps;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    if (st != GFFT_OK) break;
+    set_variant(c, axis, best);
+    a->tuned_ms = best_ms;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
+}
+
+// z and y passes over a block of `planes` consecutive local x-planes,
+// alternating per block so that a block's z output is still in the
+// memory-side cache (MALL) when its y pass reads it.
+int exec_zy_chunked(const PlanCore& c, bool inverse, void* data, hipStream_t s) {
+  const int64_t nxl = c.nx / c.procs;
+  const size_t rs = c.prec == F32 ? 4 : 8;
+  const double zs = axis_scale(c, 0, inverse), ys = axis_scale(c, 1, inverse);
+  for (int64_t x0 = 0; x0 < nxl; x0 += c.chunk_planes) {
+    const int64_t B = std::min<int64_t>(c.chunk_planes, nxl - x0);
+    char* base = static_cast<char*>(data) + (size_t)x0 * c.ny * 2 * c.K * rs;
+    RowGeom zg = z_geom(c, zs);
+    zg.nrows = B * c.ny;
+    ColGeom yg = y_geom(c, ys);
+    yg.ncols = (uint32_t)(B * c.K);
+    hipError_t e;
+    if (!inverse) {
+      if ((e = run_rows(c, c.z, false, base, zg, s)) != hipSuccess) return GFFT_ERR_HIP;
+      if ((e = run_cols(c, c.y, false, base, yg, s)) != hipSuccess) return GFFT_ERR_HIP;
+    } else {
+      if ((e = run_cols(c, c.y, true, base, yg, s)) != hipSuccess) return GFFT_ERR_HIP;
+      if ((e = run_rows(c, c.z, true, base, zg, s)) != hipSuccess) return GFFT_ERR_HIP;
+    }
+  }
+  return GFFT_OK;
+}
+
+bool use_chunks(const PlanCore& c) { return c.chunk_planes > 0 && c.chunk_planes < c.nx / c.procs; }
+
 int exec_r2c(const PlanCore& c, void* data, hipStream_t s) {
+  if (use_chunks(c)) {
+    const int st = exec_zy_chunked(c, false, data, s);
+    if (st != GFFT_OK) return st;
+    return exec_axis(c, 2, false, data, s);
+  }
   for (int axis = 0; axis < 3; ++axis) {
     const int st = exec_axis(c, axis, false, data, s);
     if (st != GFFT_OK) return st;
@@ -309,6 +474,11 @@ int exec_r2c(const PlanCore& c, void* data, hipStream_t s) {
 }
 
 int exec_c2r(const PlanCore& c, void* data, hipStream_t s) {
+  if (use_chunks(c)) {
+    const int st = exec_axis(c, 2, true, data, s);
+    if (st != GFFT_OK) return st;
+    return exec_zy_chunked(c, true, data, s);
+  }
   for (int axis = 2; axis >= 0; --axis) {
     const int st = exec_axis(c, axis, true, data, s);
     if (st != GFFT_OK) return st;

@@ -2,6 +2,8 @@
 // (gfft_abi.cpp).  Internal.
 #pragma once
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "gfft_internal.h"
 #include "gpufft/gpufft.h"
@@ -12,8 +14,12 @@ struct AxisPlan {
   int N = 0;
   bool rows = false;     // z pass (two-for-one real rows) vs strided columns
   bool active = false;   // false for a length-1 strided axis (identity)
-  const KernelEntry* reg_fwd = nullptr;  // compiled variants (both or neither)
-  const KernelEntry* reg_inv = nullptr;
+  const KernelEntry* reg_fwd = nullptr;  // chosen compiled variant (both or neither;
+  const KernelEntry* reg_inv = nullptr;  //  null = the generic run-time-schedule kernel)
+  // candidate (forward, inverse) pairs sharing `sched`; {null, null} = generic
+  std::vector<std::pair<const KernelEntry*, const KernelEntry*>> cands;
+  int variant = 0;       // index into cands
+  double tuned_ms = -1;  // fwd+inv time of the chosen variant when tuned
   Schedule sched{};      // radix order used (fixes the twiddle table)
   RtSchedHost rt{};      // the same order for the generic kernels
   void* d_tw = nullptr;  // device twiddle table
@@ -24,6 +30,7 @@ struct PlanCore {
   int64_t nx = 0, ny = 0, nz = 0, K = 0;
   int procs = 1, rank = 0, prec = 0;
   double r2c_scale = 1.0, c2r_scale = 1.0;  // fused output scales (1 = unnormalised)
+  int64_t chunk_planes = 0;  // > 0: z and y passes alternate per block of x-planes
   AxisPlan z, y, x;
 };
 
@@ -35,6 +42,8 @@ void free_twiddles(PlanCore& c);
 int exec_r2c(const PlanCore& c, void* data, hipStream_t s);
 int exec_c2r(const PlanCore& c, void* data, hipStream_t s);
 int exec_axis(const PlanCore& c, int axis, bool inverse, void* data, hipStream_t s);
+int set_variant(PlanCore& c, int axis, int variant);
+int tune(PlanCore& c, void* scratch, hipStream_t s, int reps);
 ColGeom y_geom(const PlanCore& c, double scale);
 ColGeom x_geom(const PlanCore& c, double scale);
 RowGeom z_geom(const PlanCore& c, double scale);

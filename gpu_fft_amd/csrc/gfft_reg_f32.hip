@@ -16,10 +16,41 @@ int registry_f32(const KernelEntry** out) {
       GFFT_SIZE(float, Sc192, 24, 16, 24, 16),
       GFFT_SIZE(float, Sc256, 8, 8, 8, 16),
       GFFT_SIZE(float, Sc384, 24, 8, 24, 16),
+      // 512: measured defaults first (z: 8 pairs per workgroup; y and x: persistent,
+      // pipelined, one workgroup per CU, round-robin tiles)
+      GFFT_ROWS(float, Sc512, 8, 8),
+      GFFT_COLS_PIPE(float, Sc512, 16, 16, 1, 1),
       GFFT_SIZE(float, Sc512, 8, 4, 8, 16),
       GFFT_SIZE(float, Sc768, 24, 4, 24, 16),
       GFFT_SIZE(float, Sc1024, 8, 2, 16, 16),
       GFFT_SIZE(float, Sc2048, 8, 1, 16, 8),
+      // tuning alternatives
+      GFFT_COLS(float, Sc512, 16, 16),
+      GFFT_COLS(float, Sc512, 8, 8),
+      GFFT_COLS(float, Sc512, 16, 8),
+      GFFT_COLS(float, Sc512, 32, 16),
+      GFFT_COLS(float, Sc512, 16, 32),
+      GFFT_ROWS(float, Sc512, 8, 2),
+      GFFT_ROWS(float, Sc512, 16, 4),
+      GFFT_ROWS(float, Sc512, 16, 8),
+      GFFT_COLS(float, Sc1024, 16, 8),
+      GFFT_COLS(float, Sc1024, 32, 16),
+      GFFT_COLS(float, Sc256, 16, 16),
+      GFFT_COLS(float, Sc256, 8, 8),
+      GFFT_COLS(float, Sc768, 24, 8),
+      GFFT_COLS_PIPE(float, Sc512, 8, 16, 2, 0),
+      GFFT_COLS_PIPE(float, Sc512, 16, 16, 2, 0),
+      GFFT_COLS_PIPE(float, Sc512, 16, 16, 1, 0),
+      GFFT_COLS_PIPE(float, Sc512, 32, 16, 2, 0),
+      GFFT_COLS_PIPE(float, Sc512, 8, 16, 2, 1),
+      GFFT_COLS_PIPE(float, Sc512, 16, 16, 2, 1),
+      GFFT_COLS_PIPE(float, Sc512, 32, 16, 2, 1),
+      GFFT_ROWS_PIPE(float, Sc512, 8, 4, 2, 0),
+      GFFT_ROWS_PIPE(float, Sc512, 8, 4, 4, 0),
+      GFFT_ROWS_PIPE(float, Sc512, 8, 4, 2, 1),
+      GFFT_ROWS_PIPE(float, Sc512, 8, 4, 4, 1),
+      GFFT_ROWS_PIPE(float, Sc512, 8, 8, 2, 1),
+      GFFT_ROWS_PIPE(float, Sc512, 16, 8, 2, 1),
   };
   *out = table;
   return (int)(sizeof(table) / sizeof(table[0]));

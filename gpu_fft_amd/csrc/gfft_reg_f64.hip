@@ -16,10 +16,26 @@ int registry_f64(const KernelEntry** out) {
       GFFT_SIZE(double, Sc192, 24, 8, 24, 8),
       GFFT_SIZE(double, Sc256, 8, 8, 8, 8),
       GFFT_SIZE(double, Sc384, 24, 4, 24, 8),
-      GFFT_SIZE(double, Sc512, 8, 4, 8, 8),
+      // 512: measured default columns first (16 points per thread)
+      GFFT_ROWS(double, Sc512, 8, 4),
+      GFFT_COLS(double, Sc512, 16, 8),
+      GFFT_COLS(double, Sc512, 8, 8),
       GFFT_SIZE(double, Sc768, 24, 2, 24, 8),
       GFFT_SIZE(double, Sc1024, 8, 2, 16, 8),
       GFFT_SIZE(double, Sc2048, 8, 1, 16, 4),
+      // tuning alternatives
+      GFFT_COLS(double, Sc512, 8, 4),
+      GFFT_COLS(double, Sc512, 16, 4),
+      GFFT_ROWS(double, Sc512, 8, 2),
+      GFFT_ROWS(double, Sc512, 16, 4),
+      GFFT_COLS(double, Sc768, 24, 4),
+      GFFT_COLS_PIPE(double, Sc512, 8, 8, 2, 0),
+      GFFT_COLS_PIPE(double, Sc512, 16, 8, 2, 0),
+      GFFT_COLS_PIPE(double, Sc512, 16, 8, 1, 0),
+      GFFT_COLS_PIPE(double, Sc512, 8, 8, 2, 1),
+      GFFT_COLS_PIPE(double, Sc512, 16, 8, 2, 1),
+      GFFT_ROWS_PIPE(double, Sc512, 8, 4, 2, 1),
+      GFFT_ROWS_PIPE(double, Sc512, 8, 2, 4, 1),
   };
   *out = table;
   return (int)(sizeof(table) / sizeof(table[0]));

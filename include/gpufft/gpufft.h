@@ -93,6 +93,25 @@ int gfft_exec_c2r(gfft_plan plan, void* data, gfft_stream_t stream);
  * hipFFT plans, GPU_FFT_init.cu:86-91). */
 int gfft_exec_axis(gfft_plan plan, int axis, int inverse, void* data, gfft_stream_t stream);
 
+/* Autotuner: time every compiled variant of every axis pass (tile width,
+ * points per thread, rows per workgroup, and the run-time-schedule kernel)
+ * on `scratch` (>= gfft_plan_data_bytes bytes, contents destroyed), `reps`
+ * forward+inverse pairs each, and keep the fastest.  Choices are cached per
+ * geometry for the life of the process.  Synchronises `stream`.  The
+ * reference delegates this choice to hipFFT's planner (GPU_FFT_init.cu:86-91). */
+int gfft_plan_tune(gfft_plan plan, void* scratch, gfft_stream_t stream, int reps);
+
+/* Select variant `variant` (0 = default; gfft_plan_describe lists the
+ * count as "candidates") for axis 0 = z, 1 = y, 2 = x. */
+int gfft_plan_set_variant(gfft_plan plan, int axis, int variant);
+
+/* Cache blocking of the z and y passes: with planes > 0 (and < Nx/p) the
+ * R2C runs z then y on each block of `planes` consecutive x-planes before
+ * the x pass, and the C2R runs the x pass, then y then z per block, so a
+ * block's intermediate can be served from the memory-side cache.  0 (the
+ * default) runs whole-array passes.  Same results either way. */
+int gfft_plan_set_chunking(gfft_plan plan, int64_t planes);
+
 /* Output scale factors fused into the last pass of each direction (default
  * 1.0 and 1.0: unnormalised, the reference's semantics).  c2r_scale =
  * 1/(Nx*Ny*Nz) gives the normalised inverse that the reference harness
