@@ -74,6 +74,26 @@ def cpu_baseline(n: int, reps: int, workers: int) -> dict:
     }
 
 
+def measured_traffic(pass_name: str, n: int, prec: str):
+    """HBM bytes per launch of the pass's kernel family from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a plain copy
+    of known size); None when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{n}cube_{prec}_passes.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        doc = json.load(fh)
+    fam = {"z_r2c": "k_rows_r2c", "z_c2r": "k_rows_c2r"}.get(pass_name, "k_cols")
+    inv = pass_name.endswith("_inv")
+    for name, v in doc["kernels"].items():
+        if not name.startswith(fam):
+            continue
+        if fam == "k_cols" and (", true," in name) != inv:
+            continue
+        return int(v["read_bytes"] + v["write_bytes"])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +192,7 @@ def main():
     dom = max(per_pass, key=lambda k: per_pass[k]["ms_mean"])
     dom_ms = per_pass[dom]["ms_mean"]
     achieved = 2 * S / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom, n, args.prec)
 
     # sanity: the array must still hold the (normalised) input's magnitude
     rms = float(buf[:, :, :nz].float().pow(2).mean().sqrt())
@@ -198,7 +219,7 @@ def main():
                    "nx": nx, "ny": ny, "nz": nz, "procs": 1, "precision": args.prec,
                    "inverse_scale": "1/(Nx*Ny*Nz) fused"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": 2 * S},
         "roofline_round_trip": {"alg_bytes": total_alg, "model": "12*S (3 passes x read+write per direction)",
                                 "achieved_GBps": round(total_alg / (ms_step * 1e-3) / 1e9, 1),
