@@ -66,6 +66,10 @@ def lib() -> ctypes.CDLL:
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [vp, vp, vp]
+    for name in ("gfft_exec_r2c_local", "gfft_exec_r2c_finish", "gfft_exec_c2r_start", "gfft_exec_c2r_local"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [vp, vp, vp, vp]
     L.gfft_plan_tune.restype = ctypes.c_int
     L.gfft_plan_tune.argtypes = [vp, vp, vp, ctypes.c_int]
     L.gfft_plan_set_variant.restype = ctypes.c_int
@@ -164,6 +168,23 @@ class Plan:
 
     def c2r(self, data, stream=None) -> None:
         _check(lib().gfft_exec_c2r(self.handle, _ptr(data), _stream_handle(stream)), "gfft_exec_c2r")
+
+    # local halves around a slab exchange (see gpufft.h)
+    def r2c_local(self, data, sendbuf, stream=None) -> None:
+        _check(lib().gfft_exec_r2c_local(self.handle, _ptr(data), _ptr(sendbuf), _stream_handle(stream)),
+               "gfft_exec_r2c_local")
+
+    def r2c_finish(self, recvbuf, out, stream=None) -> None:
+        _check(lib().gfft_exec_r2c_finish(self.handle, _ptr(recvbuf), _ptr(out), _stream_handle(stream)),
+               "gfft_exec_r2c_finish")
+
+    def c2r_start(self, data, sendbuf, stream=None) -> None:
+        _check(lib().gfft_exec_c2r_start(self.handle, _ptr(data), _ptr(sendbuf), _stream_handle(stream)),
+               "gfft_exec_c2r_start")
+
+    def c2r_local(self, recvbuf, data, stream=None) -> None:
+        _check(lib().gfft_exec_c2r_local(self.handle, _ptr(recvbuf), _ptr(data), _stream_handle(stream)),
+               "gfft_exec_c2r_local")
 
     def tune(self, scratch, stream=None, reps: int = 5) -> dict:
         """Autotune every axis on ``scratch`` (contents destroyed); returns

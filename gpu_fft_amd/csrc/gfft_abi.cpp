@@ -101,6 +101,34 @@ int gfft_exec_axis(gfft_plan plan, int axis, int inverse, void* data, gfft_strea
   return gfft::exec_axis(plan->core, axis, inverse != 0, data, reinterpret_cast<hipStream_t>(stream));
 }
 
+int gfft_exec_r2c_local(gfft_plan plan, void* data, void* sendbuf, gfft_stream_t stream) {
+  if (!plan || !data || !sendbuf || data == sendbuf) return GFFT_ERR_INVALID_ARG;
+  const int st = check_device(plan);
+  if (st != GFFT_OK) return st;
+  return gfft::exec_r2c_local(plan->core, data, sendbuf, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gfft_exec_r2c_finish(gfft_plan plan, const void* recvbuf, void* out, gfft_stream_t stream) {
+  if (!plan || !recvbuf || !out || recvbuf == out) return GFFT_ERR_INVALID_ARG;
+  const int st = check_device(plan);
+  if (st != GFFT_OK) return st;
+  return gfft::exec_r2c_finish(plan->core, recvbuf, out, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gfft_exec_c2r_start(gfft_plan plan, const void* data, void* sendbuf, gfft_stream_t stream) {
+  if (!plan || !data || !sendbuf || data == sendbuf) return GFFT_ERR_INVALID_ARG;
+  const int st = check_device(plan);
+  if (st != GFFT_OK) return st;
+  return gfft::exec_c2r_start(plan->core, data, sendbuf, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gfft_exec_c2r_local(gfft_plan plan, const void* recvbuf, void* data, gfft_stream_t stream) {
+  if (!plan || !recvbuf || !data || recvbuf == data) return GFFT_ERR_INVALID_ARG;
+  const int st = check_device(plan);
+  if (st != GFFT_OK) return st;
+  return gfft::exec_c2r_local(plan->core, recvbuf, data, reinterpret_cast<hipStream_t>(stream));
+}
+
 int gfft_plan_set_variant(gfft_plan plan, int axis, int variant) {
   if (!plan) return GFFT_ERR_INVALID_ARG;
   return gfft::set_variant(plan->core, axis, variant);

@@ -155,8 +155,17 @@ __global__ __launch_bounds__(kGenThreads) void k_gen_rows_c2r(T* data, RowGeom g
   }
 }
 
+__device__ __forceinline__ int64_t gen_base(const ColMap& m, uint32_t c) {
+  return (int64_t)(c / m.inner) * m.outer + (int64_t)(c % m.inner);
+}
+__device__ __forceinline__ int64_t gen_row(const ColMap& m, int n, int N) {
+  if (m.nsub >= N) return (int64_t)n * m.rlo;
+  return (int64_t)(n % m.nsub) * m.rlo + (int64_t)(n / m.nsub) * m.rhi;
+}
+
 template <typename T, bool INV>
-__global__ __launch_bounds__(kGenThreads) void k_gen_cols(cx<T>* data, ColGeom g, RtSched sc, const cx<T>* tw) {
+__global__ __launch_bounds__(kGenThreads) void k_gen_cols(const cx<T>* in, cx<T>* out, ColGeom g, RtSched sc,
+                                                             const cx<T>* tw) {
   constexpr int CW = GenCfg<T>::CW;
   extern __shared__ unsigned char raw[];
   cx<T>* buf0 = reinterpret_cast<cx<T>*>(raw);
@@ -168,8 +177,7 @@ __global__ __launch_bounds__(kGenThreads) void k_gen_cols(cx<T>* data, ColGeom g
     const uint32_t col = tile * CW + c;
     cx<T> v = mk<T>(0, 0);
     if (col < g.ncols) {
-      const int64_t base = (int64_t)(col / g.inner) * g.outer + (int64_t)(col % g.inner);
-      v = data[base + (int64_t)n * g.rstride];
+      v = in[gen_base(g.in, col) + gen_row(g.in, n, N)];
     }
     buf0[f] = v;
   }
@@ -180,9 +188,8 @@ __global__ __launch_bounds__(kGenThreads) void k_gen_cols(cx<T>* data, ColGeom g
     const int c = f % CW, n = f / CW;
     const uint32_t col = tile * CW + c;
     if (col < g.ncols) {
-      const int64_t base = (int64_t)(col / g.inner) * g.outer + (int64_t)(col % g.inner);
       const cx<T> z = Z[f];
-      data[base + (int64_t)n * g.rstride] = mk<T>(z.x * (T)g.scale, z.y * (T)g.scale);
+      out[gen_base(g.out, col) + gen_row(g.out, n, N)] = mk<T>(z.x * (T)g.scale, z.y * (T)g.scale);
     }
   }
 }
@@ -245,7 +252,8 @@ size_t cols_lds_bytes(int N) {
 }
 
 template <typename T>
-hipError_t gen_cols(int kind, void* data, ColGeom g, const RtSchedHost& h, const void* tw, hipStream_t s) {
+hipError_t gen_cols(int kind, const void* in, void* out, ColGeom g, const RtSchedHost& h, const void* tw,
+                    hipStream_t s) {
   constexpr int CW = GenCfg<T>::CW;
   const int64_t blocks = ((int64_t)g.ncols + CW - 1) / CW;
   if (blocks <= 0) return hipSuccess;
@@ -258,10 +266,10 @@ hipError_t gen_cols(int kind, void* data, ColGeom g, const RtSchedHost& h, const
   if (e != hipSuccess) return e;
   if (kind == COLS_INV)
     hipLaunchKernelGGL((k_gen_cols<T, true>), dim3((uint32_t)blocks), dim3(kGenThreads), lds, s,
-                       static_cast<cx<T>*>(data), g, sc, static_cast<const cx<T>*>(tw));
+                       static_cast<const cx<T>*>(in), static_cast<cx<T>*>(out), g, sc, static_cast<const cx<T>*>(tw));
   else
     hipLaunchKernelGGL((k_gen_cols<T, false>), dim3((uint32_t)blocks), dim3(kGenThreads), lds, s,
-                       static_cast<cx<T>*>(data), g, sc, static_cast<const cx<T>*>(tw));
+                       static_cast<const cx<T>*>(in), static_cast<cx<T>*>(out), g, sc, static_cast<const cx<T>*>(tw));
   return hipGetLastError();
 }
 
@@ -272,9 +280,10 @@ hipError_t launch_generic_rows(int prec, int kind, void* data, RowGeom g, const 
   return prec == F32 ? gen_rows<float>(kind, data, g, sc, tw, s) : gen_rows<double>(kind, data, g, sc, tw, s);
 }
 
-hipError_t launch_generic_cols(int prec, int kind, void* data, ColGeom g, const RtSchedHost& sc, const void* tw,
+hipError_t launch_generic_cols(int prec, int kind, const void* in, void* out, ColGeom g, const RtSchedHost& sc,
+                               const void* tw,
                                hipStream_t s) {
-  return prec == F32 ? gen_cols<float>(kind, data, g, sc, tw, s) : gen_cols<double>(kind, data, g, sc, tw, s);
+  return prec == F32 ? gen_cols<float>(kind, in, out, g, sc, tw, s) : gen_cols<double>(kind, in, out, g, sc, tw, s);
 }
 
 size_t generic_rows_lds(int prec, int N, int K) {

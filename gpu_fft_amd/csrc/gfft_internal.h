@@ -18,17 +18,35 @@ enum Kind {
   NUM_KINDS = 4
 };
 
-// Column geometry of a strided pass: column c (0 <= c < ncols) starts at
-// element (c / inner) * outer + (c % inner); its point n sits rstride
-// elements further per step.  y pass: inner = K, outer = Ny*K, rstride = K.
-// x pass: inner = ncols = Ny*K, outer = 0, rstride = Ny*K.
+// Address map of one side (input or output) of a strided pass: column c
+// starts at element (c / inner) * outer + (c % inner); its point n sits at
+// + (n % nsub) * rlo + (n / nsub) * rhi (nsub >= N: a single stride rlo).
+//   y pass, in place:  inner = K, outer = Ny*K, rlo = K
+//   x pass, in place:  inner = Ny*K, outer = 0, rlo = Ny*K
+// The two-level form addresses the p destination chunks of a slab exchange.
+struct ColMap {
+  uint32_t inner;
+  int32_t nsub;
+  int64_t outer;
+  int64_t rlo;
+  int64_t rhi;
+};
+
 struct ColGeom {
   uint32_t ncols;
-  uint32_t inner;
-  int64_t outer;
-  int64_t rstride;
+  ColMap in, out;
   double scale;  // applied to the outputs (1 = unnormalised)
 };
+
+inline ColMap col_map(uint32_t inner, int64_t outer, int64_t rlo) {
+  ColMap m;
+  m.inner = inner;
+  m.nsub = 0x7fffffff;
+  m.outer = outer;
+  m.rlo = rlo;
+  m.rhi = 0;
+  return m;
+}
 
 // Geometry of a z pass: nrows real rows of Nz points; row r's reals start at
 // r * 2K and its K half-spectrum bins at complex index r * K (same bytes).
@@ -49,7 +67,9 @@ struct Schedule {
 // Launch signatures.  `tw` is the stage twiddle table of the kernel's
 // schedule (see gfft_twiddles.cpp), in the transform's precision.
 typedef hipError_t (*RowsLaunchFn)(void* data, RowGeom g, const void* tw, hipStream_t s);
-typedef hipError_t (*ColsLaunchFn)(void* data, ColGeom g, const void* tw, hipStream_t s);
+// Column passes read `in` and write `out` (the same pointer when in place; a
+// workgroup reads all of its tile before writing it, tiles are disjoint).
+typedef hipError_t (*ColsLaunchFn)(const void* in, void* out, ColGeom g, const void* tw, hipStream_t s);
 
 // A compiled (register-resident) kernel variant.
 struct KernelEntry {
@@ -79,7 +99,7 @@ struct RtSchedHost {
 };
 hipError_t launch_generic_rows(int prec, int kind, void* data, RowGeom g, const RtSchedHost& sc,
                                const void* tw, hipStream_t s);
-hipError_t launch_generic_cols(int prec, int kind, void* data, ColGeom g, const RtSchedHost& sc,
+hipError_t launch_generic_cols(int prec, int kind, const void* in, void* out, ColGeom g, const RtSchedHost& sc,
                                const void* tw, hipStream_t s);
 // LDS bytes the generic kernels would need for a size (0 if unsupported).
 size_t generic_rows_lds(int prec, int N, int K);

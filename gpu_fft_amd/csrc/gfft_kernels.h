@@ -207,40 +207,61 @@ struct RowsC2R {
 // ---------------------------------------------------------------------------
 // Strided C2C column pass, register engine
 // ---------------------------------------------------------------------------
-template <typename T, bool INV, typename SC, int P, int CW>
+__device__ __forceinline__ int64_t map_base(const ColMap& m, uint32_t c) {
+  return (int64_t)(c / m.inner) * m.outer + (int64_t)(c % m.inner);
+}
+template <int N>
+__device__ __forceinline__ int64_t map_row(const ColMap& m, int n) {
+  if (m.nsub >= N) return (int64_t)n * m.rlo;  // uniform branch: single stride
+  return (int64_t)(n % m.nsub) * m.rlo + (int64_t)(n / m.nsub) * m.rhi;
+}
+
+// LB selects LDS-only stage barriers (persistent, prefetching kernels).
+template <typename T, bool INV, typename SC, int P, int CW, bool LB = false>
 struct ColsPass {
-  using E = RegEngine<T, INV, SC, P>;
+  using E = RegEngine<T, INV, SC, P, LB>;
   using C = cx<T>;
   static constexpr int N = SC::N;
   static constexpr int TPT = E::TPT;
   static constexpr int NT = CW * TPT;
 
-  __device__ __forceinline__ static void tile(C* __restrict__ data, ColGeom g, const C* __restrict__ tw,
-                                              uint32_t tile_id, C* smem) {
-    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
-    const uint32_t col = tile_id * CW + c;
+  // `in` and `out` may be the same array (in-place pass): no __restrict__
+  __device__ __forceinline__ static void load(C (&v)[P], const C* in, const ColGeom& g, uint32_t tile,
+                                              int c, int t) {
+    const uint32_t col = tile * CW + c;
     const bool valid = col < g.ncols;
-    const int64_t base = valid ? (int64_t)(col / g.inner) * g.outer + (int64_t)(col % g.inner) : 0;
-    C* p = data + base;
-    C v[P];
+    const C* p = in + (valid ? map_base(g.in, col) : 0);
 #pragma unroll
     for (int q = 0; q < P / E::R0; ++q)
 #pragma unroll
       for (int r = 0; r < E::R0; ++r)
-        v[q * E::R0 + r] = valid ? ld_stream(p + (int64_t)E::in_index(t, q, r) * g.rstride) : mk<T>(0, 0);
-    ColLds<T, CW> L{smem, c};
-    E::run(v, t, L, tw);
+        v[q * E::R0 + r] = valid ? ld_stream(p + map_row<N>(g.in, E::in_index(t, q, r))) : mk<T>(0, 0);
+  }
+
+  __device__ __forceinline__ static void store(C (&v)[P], C* out, const ColGeom& g, uint32_t tile, int c,
+                                               int t) {
+    const uint32_t col = tile * CW + c;
+    if (col >= g.ncols) return;
+    C* p = out + map_base(g.out, col);
     if (g.scale != 1.0) {
       const T sc = (T)g.scale;
 #pragma unroll
       for (int i = 0; i < P; ++i) v[i] = mk<T>(v[i].x * sc, v[i].y * sc);
     }
-    if (valid) {
 #pragma unroll
-      for (int q = 0; q < P / E::RL; ++q)
+    for (int q = 0; q < P / E::RL; ++q)
 #pragma unroll
-        for (int r = 0; r < E::RL; ++r) st_stream(p + (int64_t)E::out_index(t, q, r) * g.rstride, v[q * E::RL + r]);
-    }
+      for (int r = 0; r < E::RL; ++r) st_stream(p + map_row<N>(g.out, E::out_index(t, q, r)), v[q * E::RL + r]);
+  }
+
+  __device__ __forceinline__ static void tile(const C* in, C* out, const ColGeom& g, const C* __restrict__ tw,
+                                              uint32_t tile_id, C* smem) {
+    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+    C v[P];
+    load(v, in, g, tile_id, c, t);
+    ColLds<T, CW> L{smem, c};
+    E::run(v, t, L, tw);
+    store(v, out, g, tile_id, c, t);
   }
 };
 
@@ -264,11 +285,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsC2R<T, SC, P, G>::N
 }
 
 template <typename T, bool INV, typename SC, int P, int CW>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPass<T, INV, SC, P, CW>::NT))) void k_cols(cx<T>* data, ColGeom g, const cx<T>* tw) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPass<T, INV, SC, P, CW>::NT))) void k_cols(
+    const cx<T>* in, cx<T>* out, ColGeom g, const cx<T>* tw) {
   using K_ = ColsPass<T, INV, SC, P, CW>;
   __shared__ cx<T> smem[SC::N * CW];
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  K_::tile(data, g, tw, tile, smem);
+  K_::tile(in, out, g, tw, tile, smem);
 }
 
 // Persistent, software-pipelined column pass.  Workgroup w owns the
@@ -277,43 +299,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPass<T, INV, SC, P,
 // tile i+1 are already in flight in a second register set.  Stage barriers
 // wait for LDS traffic only, so the prefetch is not drained by them.
 template <typename T, bool INV, typename SC, int P, int CW>
-struct ColsPipe {
-  using E = RegEngine<T, INV, SC, P, true>;
-  using C = cx<T>;
-  static constexpr int N = SC::N;
-  static constexpr int TPT = E::TPT;
-  static constexpr int NT = CW * TPT;
-
-  __device__ __forceinline__ static void load(C (&v)[P], const C* __restrict__ data, const ColGeom& g, uint32_t tile,
-                                              int c, int t) {
-    const uint32_t col = tile * CW + c;
-    const bool valid = col < g.ncols;
-    const int64_t base = valid ? (int64_t)(col / g.inner) * g.outer + (int64_t)(col % g.inner) : 0;
-    const C* p = data + base;
-#pragma unroll
-    for (int q = 0; q < P / E::R0; ++q)
-#pragma unroll
-      for (int r = 0; r < E::R0; ++r)
-        v[q * E::R0 + r] = valid ? ld_stream(p + (int64_t)E::in_index(t, q, r) * g.rstride) : mk<T>(0, 0);
-  }
-
-  __device__ __forceinline__ static void store(C (&v)[P], C* __restrict__ data, const ColGeom& g, uint32_t tile, int c,
-                                               int t) {
-    const uint32_t col = tile * CW + c;
-    if (col >= g.ncols) return;
-    const int64_t base = (int64_t)(col / g.inner) * g.outer + (int64_t)(col % g.inner);
-    C* p = data + base;
-    if (g.scale != 1.0) {
-      const T sc = (T)g.scale;
-#pragma unroll
-      for (int i = 0; i < P; ++i) v[i] = mk<T>(v[i].x * sc, v[i].y * sc);
-    }
-#pragma unroll
-    for (int q = 0; q < P / E::RL; ++q)
-#pragma unroll
-      for (int r = 0; r < E::RL; ++r) st_stream(p + (int64_t)E::out_index(t, q, r) * g.rstride, v[q * E::RL + r]);
-  }
-};
+using ColsPipe = ColsPass<T, INV, SC, P, CW, true>;
 
 // Tiles of a persistent workgroup: ORD 0 = the contiguous range
 // [w*T/G, (w+1)*T/G); ORD 1 = round robin w, w+G, w+2G, ... (neighbouring
@@ -352,7 +338,7 @@ constexpr int twiddle_count() {
 
 template <typename T, bool INV, typename SC, int P, int CW, int ORD>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPipe<T, INV, SC, P, CW>::NT))) void k_cols_pipe(
-    cx<T>* data, ColGeom g, const cx<T>* tw, uint32_t ntiles) {
+    const cx<T>* in, cx<T>* out, ColGeom g, const cx<T>* tw, uint32_t ntiles) {
   using K_ = ColsPipe<T, INV, SC, P, CW>;
   __shared__ cx<T> smem[SC::N * CW];
   __shared__ cx<T> stw[twiddle_count<SC>()];
@@ -362,15 +348,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPipe<T, INV, SC, P,
   const int c = threadIdx.x % CW, t = threadIdx.x / CW;
   ColLds<T, CW> L{smem, c};
   cx<T> cur[P], nxt[P];
-  K_::load(nxt, data, g, walk.at(0), c, t);
+  K_::load(nxt, in, g, walk.at(0), c, t);
   for (uint32_t k = 0; k < walk.count; ++k) {
 #pragma unroll
     for (int i = 0; i < P; ++i) cur[i] = nxt[i];
-    if (k + 1 < walk.count) K_::load(nxt, data, g, walk.at(k + 1), c, t);
+    if (k + 1 < walk.count) K_::load(nxt, in, g, walk.at(k + 1), c, t);
     // the first stage uses no twiddles; two barriers precede the first
     // twiddle read, so the LDS copy above is complete by then
     K_::E::run(cur, t, L, stw);
-    K_::store(cur, data, g, walk.at(k), c, t);
+    K_::store(cur, out, g, walk.at(k), c, t);
   }
 }
 
@@ -449,11 +435,11 @@ hipError_t launch_rows_c2r(void* data, RowGeom g, const void* tw, hipStream_t s)
 }
 
 template <typename T, bool INV, typename SC, int P, int CW>
-hipError_t launch_cols(void* data, ColGeom g, const void* tw, hipStream_t s) {
+hipError_t launch_cols(const void* in, void* out, ColGeom g, const void* tw, hipStream_t s) {
   const int64_t blocks = ((int64_t)g.ncols + CW - 1) / CW;
   if (blocks <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_cols<T, INV, SC, P, CW>), dim3((uint32_t)blocks), dim3(ColsPass<T, INV, SC, P, CW>::NT), 0,
-                     s, static_cast<cx<T>*>(data), g, static_cast<const cx<T>*>(tw));
+                     s, static_cast<const cx<T>*>(in), static_cast<cx<T>*>(out), g, static_cast<const cx<T>*>(tw));
   return hipGetLastError();
 }
 
@@ -463,11 +449,12 @@ inline int64_t persistent_blocks(int64_t ntiles, int occ) {
 }
 
 template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD>
-hipError_t launch_cols_pipe(void* data, ColGeom g, const void* tw, hipStream_t s) {
+hipError_t launch_cols_pipe(const void* in, void* out, ColGeom g, const void* tw, hipStream_t s) {
   const int64_t ntiles = ((int64_t)g.ncols + CW - 1) / CW;
   if (ntiles <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_cols_pipe<T, INV, SC, P, CW, ORD>), dim3((uint32_t)persistent_blocks(ntiles, OCC)),
-                     dim3(ColsPipe<T, INV, SC, P, CW>::NT), 0, s, static_cast<cx<T>*>(data), g,
+                     dim3(ColsPipe<T, INV, SC, P, CW>::NT), 0, s, static_cast<const cx<T>*>(in),
+                     static_cast<cx<T>*>(out), g,
                      static_cast<const cx<T>*>(tw), (uint32_t)ntiles);
   return hipGetLastError();
 }

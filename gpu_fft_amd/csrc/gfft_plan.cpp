@@ -276,25 +276,68 @@ void free_twiddles(PlanCore& c) {
   }
 }
 
+// y pass over the local x-slab (Nx/p, Ny, K), in place: column c = (x, kz).
 ColGeom y_geom(const PlanCore& c, double scale) {
   const int64_t nxl = c.nx / c.procs;
   ColGeom g;
   g.ncols = (uint32_t)(nxl * c.K);
-  g.inner = (uint32_t)c.K;
-  g.outer = c.ny * c.K;
-  g.rstride = c.K;
+  g.in = g.out = col_map((uint32_t)c.K, c.ny * c.K, c.K);
   g.scale = scale;
   return g;
 }
 
+// x pass over the y-slab (Nx, Ny/p, K), in place: column c = (y, kz).
 ColGeom x_geom(const PlanCore& c, double scale) {
   const int64_t nyl = c.ny / c.procs;
   ColGeom g;
   g.ncols = (uint32_t)(nyl * c.K);
-  g.inner = g.ncols;
-  g.outer = 0;
-  g.rstride = nyl * c.K;
+  g.in = g.out = col_map(g.ncols, 0, nyl * c.K);
   g.scale = scale;
+  return g;
+}
+
+// The slab-exchange buffers hold p chunks of (Ny/p, Nx/p, K), chunk s for
+// (or from) rank s: the reference's __buffer__ after __transpose_slab__
+// (transpose.cu:41-54, FFT_R2C.cu:48) and the receive layout that
+// __chunk_transpose__ unpacks (transpose.cu:56-73, FFT_R2C.cu:66).
+//
+// y pass with the pack fused into its stores: x-slab (Nx/p, Ny, K) ->
+// (Ny, Nx/p, K); column c = (xl, kz) lands at c + y * (Nx/p) * K.
+ColGeom y_pack_geom(const PlanCore& c, double scale) {
+  ColGeom g = y_geom(c, scale);
+  g.out = col_map(g.ncols, 0, (c.nx / c.procs) * c.K);
+  return g;
+}
+
+// The inverse: y pass reading (Ny, Nx/p, K), writing the x-slab in place
+// order (FFT_C2R.cu:66, __transpose_slab__ back).
+ColGeom y_unpack_geom(const PlanCore& c, double scale) {
+  ColGeom g = y_geom(c, scale);
+  g.in = col_map(g.ncols, 0, (c.nx / c.procs) * c.K);
+  return g;
+}
+
+// x pass reading the p received chunks: point x = s * (Nx/p) + xl of column
+// (yl, kz) sits at s * chunk + (yl * (Nx/p) + xl) * K + kz.
+ColMap chunk_map(const PlanCore& c) {
+  const int64_t nxl = c.nx / c.procs, nyl = c.ny / c.procs;
+  ColMap m = col_map((uint32_t)c.K, nxl * c.K, c.K);
+  m.nsub = (int32_t)nxl;
+  m.rhi = nxl * nyl * c.K;
+  return m;
+}
+
+ColGeom x_unpack_geom(const PlanCore& c, double scale) {
+  ColGeom g = x_geom(c, scale);
+  g.in = chunk_map(c);
+  return g;
+}
+
+// x inverse with the pack into p chunks fused into its stores
+// (__chunk_transpose_inverse__, transpose.cu:75-91, FFT_C2R.cu:48).
+ColGeom x_pack_geom(const PlanCore& c, double scale) {
+  ColGeom g = x_geom(c, scale);
+  g.out = chunk_map(c);
   return g;
 }
 
@@ -312,11 +355,16 @@ hipError_t run_rows(const PlanCore& c, const AxisPlan& a, bool inverse, void* da
   return launch_generic_rows(c.prec, inverse ? ROWS_C2R : ROWS_R2C, data, g, a.rt, a.d_tw, s);
 }
 
+hipError_t run_cols(const PlanCore& c, const AxisPlan& a, bool inverse, const void* in, void* out, ColGeom g,
+                    hipStream_t s) {
+  const KernelEntry* e = inverse ? a.reg_inv : a.reg_fwd;
+  if (e) return reinterpret_cast<ColsLaunchFn>(e->fn)(in, out, g, a.d_tw, s);
+  return launch_generic_cols(c.prec, inverse ? COLS_INV : COLS_FWD, in, out, g, a.rt, a.d_tw, s);
+}
+
 hipError_t run_cols(const PlanCore& c, const AxisPlan& a, bool inverse, void* data, ColGeom g, hipStream_t s) {
   if (!a.active) return hipSuccess;
-  const KernelEntry* e = inverse ? a.reg_inv : a.reg_fwd;
-  if (e) return reinterpret_cast<ColsLaunchFn>(e->fn)(data, g, a.d_tw, s);
-  return launch_generic_cols(c.prec, inverse ? COLS_INV : COLS_FWD, data, g, a.rt, a.d_tw, s);
+  return run_cols(c, a, inverse, data, data, g, s);
 }
 
 // The output scale of a direction is fused into its last active pass:
@@ -365,8 +413,8 @@ std::string tune_key(const PlanCore& c, int axis) {
     std::snprintf(b, sizeof b, "p%d:rows:%d:%lld", c.prec, c.z.N, (long long)g.nrows);
   } else {
     const ColGeom g = axis == 1 ? y_geom(c, 1.0) : x_geom(c, 1.0);
-    std::snprintf(b, sizeof b, "p%d:cols:%d:%u:%u:%lld:%lld", c.prec, axis == 1 ? c.y.N : c.x.N, g.ncols, g.inner,
-                  (long long)g.outer, (long long)g.rstride);
+    std::snprintf(b, sizeof b, "p%d:cols:%d:%u:%u:%lld:%lld", c.prec, axis == 1 ? c.y.N : c.x.N, g.ncols,
+                  g.in.inner, (long long)g.in.outer, (long long)g.in.rlo);
   }
   return b;
 }
@@ -484,6 +532,58 @@ int exec_c2r(const PlanCore& c, void* data, hipStream_t s) {
     if (st != GFFT_OK) return st;
   }
   return GFFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The two halves of each direction around a slab exchange (any p, incl. 1).
+// ---------------------------------------------------------------------------
+size_t local_bytes(const PlanCore& c) {
+  return (size_t)(c.nx / c.procs) * (size_t)c.ny * (size_t)c.K * (c.prec == F32 ? 8u : 16u);
+}
+
+namespace {
+int copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s) == hipSuccess ? GFFT_OK : GFFT_ERR_HIP;
+}
+}  // namespace
+
+// FFT_R2C.cu:45-48: z R2C and y C2C on the local x-slab, the slab transpose
+// fused into the y pass's stores (data -> sendbuf).
+int exec_r2c_local(const PlanCore& c, void* data, void* sendbuf, hipStream_t s) {
+  if (run_rows(c, c.z, false, data, z_geom(c, axis_scale(c, 0, false)), s) != hipSuccess) return GFFT_ERR_HIP;
+  if (!c.y.active) return copy_d2d(sendbuf, data, local_bytes(c), s);  // Ny = 1: same order
+  return run_cols(c, c.y, false, data, sendbuf, y_pack_geom(c, axis_scale(c, 1, false)), s) == hipSuccess
+             ? GFFT_OK
+             : GFFT_ERR_HIP;
+}
+
+// FFT_R2C.cu:66-69: the chunk transpose fused into the x pass's loads
+// (recvbuf -> data, the y-slab output).
+int exec_r2c_finish(const PlanCore& c, const void* recvbuf, void* out, hipStream_t s) {
+  if (!c.x.active) return copy_d2d(out, recvbuf, local_bytes(c), s);  // Nx = 1 (then p = 1)
+  return run_cols(c, c.x, false, recvbuf, out, x_unpack_geom(c, axis_scale(c, 2, false)), s) == hipSuccess
+             ? GFFT_OK
+             : GFFT_ERR_HIP;
+}
+
+// FFT_C2R.cu:45-48: x inverse with the chunk pack fused into its stores.
+int exec_c2r_start(const PlanCore& c, const void* data, void* sendbuf, hipStream_t s) {
+  if (!c.x.active) return copy_d2d(sendbuf, data, local_bytes(c), s);
+  return run_cols(c, c.x, true, data, sendbuf, x_pack_geom(c, axis_scale(c, 2, true)), s) == hipSuccess
+             ? GFFT_OK
+             : GFFT_ERR_HIP;
+}
+
+// FFT_C2R.cu:66-69: y inverse reading the received slab-transposed layout,
+// then z C2R in place on the x-slab.
+int exec_c2r_local(const PlanCore& c, const void* recvbuf, void* data, hipStream_t s) {
+  int st = GFFT_OK;
+  if (!c.y.active)
+    st = copy_d2d(data, recvbuf, local_bytes(c), s);
+  else if (run_cols(c, c.y, true, recvbuf, data, y_unpack_geom(c, axis_scale(c, 1, true)), s) != hipSuccess)
+    st = GFFT_ERR_HIP;
+  if (st != GFFT_OK) return st;
+  return run_rows(c, c.z, true, data, z_geom(c, axis_scale(c, 0, true)), s) == hipSuccess ? GFFT_OK : GFFT_ERR_HIP;
 }
 
 }  // namespace gfft

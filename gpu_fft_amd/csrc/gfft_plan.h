@@ -43,6 +43,11 @@ int exec_r2c(const PlanCore& c, void* data, hipStream_t s);
 int exec_c2r(const PlanCore& c, void* data, hipStream_t s);
 int exec_axis(const PlanCore& c, int axis, bool inverse, void* data, hipStream_t s);
 int set_variant(PlanCore& c, int axis, int variant);
+size_t local_bytes(const PlanCore& c);
+int exec_r2c_local(const PlanCore& c, void* data, void* sendbuf, hipStream_t s);
+int exec_r2c_finish(const PlanCore& c, const void* recvbuf, void* out, hipStream_t s);
+int exec_c2r_start(const PlanCore& c, const void* data, void* sendbuf, hipStream_t s);
+int exec_c2r_local(const PlanCore& c, const void* recvbuf, void* data, hipStream_t s);
 int tune(PlanCore& c, void* scratch, hipStream_t s, int reps);
 ColGeom y_geom(const PlanCore& c, double scale);
 ColGeom x_geom(const PlanCore& c, double scale);

@@ -85,6 +85,34 @@ int gfft_exec_r2c(gfft_plan plan, void* data, gfft_stream_t stream);
  * FFT_C2R.cu:41-70). */
 int gfft_exec_c2r(gfft_plan plan, void* data, gfft_stream_t stream);
 
+/* The local halves of a slab-decomposed transform (any procs >= 1), for a
+ * caller that performs the inter-rank exchange itself (MPI, RCCL, or copies).
+ * Exchange buffers hold p chunks of (Ny/p, Nx/p, K) complex values, chunk s
+ * for (send) or from (receive) rank s -- the reference's layout around its
+ * MPI block (FFT_R2C.cu:51-63, FFT_C2R.cu:51-63).  All buffers are
+ * gfft_plan_data_bytes long and must not alias.
+ *   R2C = gfft_exec_r2c_local -> all-to-all of chunks -> gfft_exec_r2c_finish
+ *   C2R = gfft_exec_c2r_start -> all-to-all of chunks -> gfft_exec_c2r_local */
+
+/* z R2C + y C2C on the local x-slab (in place in `data`, which is then
+ * scratch), with the slab transpose fused into the y pass's stores into
+ * `sendbuf`.  Replaces FFT_R2C.cu:45-48 (hipFFT R2C + __transpose_slab__). */
+int gfft_exec_r2c_local(gfft_plan plan, void* data, void* sendbuf, gfft_stream_t stream);
+
+/* x C2C reading the received chunks (the chunk transpose fused into its
+ * loads), writing the (Nx, Ny/p, K) y-slab to `out`.  Replaces
+ * FFT_R2C.cu:66-69 (__chunk_transpose__ + hipFFT C2C forward). */
+int gfft_exec_r2c_finish(gfft_plan plan, const void* recvbuf, void* out, gfft_stream_t stream);
+
+/* x C2C inverse of the y-slab `data`, packing into p chunks in `sendbuf`.
+ * Replaces FFT_C2R.cu:45-48 (hipFFT C2C inverse + __chunk_transpose_inverse__). */
+int gfft_exec_c2r_start(gfft_plan plan, const void* data, void* sendbuf, gfft_stream_t stream);
+
+/* y C2C inverse reading the received slab-transposed layout, then z C2R in
+ * place, leaving the real x-slab in `data`.  Replaces FFT_C2R.cu:66-69
+ * (__transpose_slab__ + hipFFT C2R). */
+int gfft_exec_c2r_local(gfft_plan plan, const void* recvbuf, void* data, gfft_stream_t stream);
+
 /* Run one axis pass of the plan in place (axis 0 = z rows, 1 = y, 2 = x;
  * inverse 0 = the R2C direction, 1 = the C2R direction), for per-pass
  * measurement and for callers that interleave their own work between passes.
