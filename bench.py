@@ -49,27 +49,28 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(n: int, reps: int, workers: int) -> dict:
+def cpu_baseline(shape, reps: int, workers: int) -> dict:
     """The FP64 oracle (pocketfft via scipy.fft, multithreaded) on the same
-    transform: R2C + C2R of an n^3 seeded field, median of ``reps`` runs."""
-    import numpy as np
+    transform: R2C + C2R of the seeded field, median of ``reps`` runs."""
     from oracle import gfft_oracle as O
-    f = O.pad_real(O.random_field(n, n, n, seed=0))
+    nx, ny, nz = shape
+    f = O.pad_real(O.random_field(nx, ny, nz, seed=0))
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        s = O.r2c_p1(f, n, workers=workers)
-        O.c2r_p1(s, n, workers=workers)
+        s = O.r2c_p1(f, nz, workers=workers)
+        O.c2r_p1(s, nz, workers=workers)
         times.append(time.perf_counter() - t0)
         del s
     t = statistics.median(times)
-    flops = 2 * 5.0 * n ** 3 * math.log2(n ** 3)
+    npts = nx * ny * nz
+    flops = 2 * 5.0 * npts * math.log2(npts)
     return {
         "value": round(flops / t / 1e9, 3),
         "unit": "GFLOP/s",
         "cores": workers,
         "kind": "port",
-        "sample": (f"FP64 oracle (scipy.fft pocketfft, {workers} threads) R2C+C2R of {n}^3, median of {reps} "
+        "sample": (f"FP64 oracle (scipy.fft pocketfft, {workers} threads) R2C+C2R of {nx}x{ny}x{nz}, median of {reps} "
                    f"runs ({t * 1e3:.1f} ms each), CPU {cpu_model()}, {os.cpu_count()} logical CPUs"),
     }
 
@@ -94,12 +95,153 @@ def measured_traffic(pass_name: str, n: int, prec: str):
     return None
 
 
+def slab_share(args, nx, ny, nz, procs):
+    """One rank's share of a slab decomposition: the local R2C half (z R2C +
+    y C2C with the slab-transpose pack fused into the y stores), on the
+    (Nx/p, Ny, 2K) block.  Algorithmic bytes: SURVEY's unfused model 6 S_local
+    (z, y, pack each read + write), the fused data movement is 4 S_local."""
+    import torch
+    import gpu_fft_amd as G
+    K = nz // 2 + 1
+    nxl = nx // procs
+    rb = 4 if args.prec == "f32" else 8
+    S = nxl * ny * K * 2 * rb
+    dev = torch.device("cuda:0")
+    dt = torch.float32 if args.prec == "f32" else torch.float64
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    data = torch.rand((nxl, ny, 2 * K), dtype=dt, device=dev, generator=gen) * 2 - 1
+    send = torch.empty_like(data)
+    plan = G.Plan(nx, ny, nz, procs=procs, rank=0, precision=args.prec)
+    if not args.no_tune:
+        plan.tune(send, reps=3)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        plan.r2c_local(data, send, stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.r2c_local(data, send, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    out = {"metric": "p-share local R2C half (z R2C + y C2C + pack) time", "value": round(ms, 5), "unit": "ms",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
+           "higher_is_better": False, "dtype": args.prec,
+           "config": {"workload": f"rank share of {nx}x{ny}x{nz} at p={procs}: {nxl}x{ny}x{nz} local block",
+                      "S_local": S},
+           "roofline": {"bound": "hbm", "alg_bytes_unfused_6S": 6 * S,
+                        "frac_unfused_model": round(6 * S / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                        "data_moved_4S_GBps": round(4 * S / (ms * 1e-3) / 1e9, 1),
+                        "frac_fused_data": round(4 * S / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
+           "variants": {ax: plan.describe()[ax] for ax in ("z", "y")}}
+    plan.close()
+    print(json.dumps(out), flush=True)
+
+
+BATCH_SIZES = [32, 48, 64, 96, 128, 160, 192, 256]
+
+
+def batch_dims(count: int, seed: int):
+    """Cube edges drawn from BATCH_SIZES with weights proportional to 1/N^3."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    w = np.array([1.0 / s ** 3 for s in BATCH_SIZES])
+    pick = rng.choice(BATCH_SIZES, size=count, p=w / w.sum())
+    return [(int(n), int(n), int(n)) for n in pick]
+
+
+def batch_bench(args):
+    """256 independent FP32 R2C+C2R round trips of uneven cube sizes: the
+    batched level kernels replayed as one hipGraph, against sequential
+    plain-method launches (one prebuilt plan per size, R2C then C2R per
+    transform, same stream, same fused 1/N scale)."""
+    import torch
+    import gpu_fft_amd as G
+    dims = batch_dims(args.batch, seed=0)
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    bufs = []
+    for nx, ny, nz in dims:
+        b = torch.rand((nx, ny, 2 * (nz // 2 + 1)), device=dev, generator=gen) * 2 - 1
+        b[:, :, nz:] = 0
+        bufs.append(b)
+    scales = [1.0 / (nx * ny * nz) for nx, ny, nz in dims]
+    s_bytes = sum(nx * ny * (nz // 2 + 1) * 8 for nx, ny, nz in dims)
+    flops = sum(2 * 5.0 * nx * ny * nz * math.log2(nx * ny * nz) for nx, ny, nz in dims)
+    stream = torch.cuda.current_stream()
+
+    plans = {}
+    for d in set(dims):
+        p = G.Plan(*d)
+        p.set_scale(1.0, 1.0 / (d[0] * d[1] * d[2]))
+        plans[d] = p
+
+    def plain():
+        for d, b in zip(dims, bufs):
+            plans[d].r2c(b, stream)
+            plans[d].c2r(b, stream)
+
+    batch = G.Batch(dims)
+    batch.set_data(bufs)
+    batch.set_scales(c2r=scales)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    ref = [b.clone() for b in bufs]
+    print(f"batch: {len(dims)} transforms, plain path", file=sys.stderr, flush=True)
+    ms_plain = timed(plain)
+    print("batch: graph path", file=sys.stderr, flush=True)
+    ms_batch = timed(lambda: batch.exec_graph(G.ROUND_TRIP, stream))
+    print("batch: eager path", file=sys.stderr, flush=True)
+    ms_batch_eager = timed(lambda: batch.exec(G.ROUND_TRIP, stream))
+    # every execution above was a normalised round trip: the fields came back
+    torch.cuda.synchronize()
+    rt_err = max(float((b - r)[..., :d[2]].abs().max() / r[..., :d[2]].abs().max())
+                 for b, r, d in zip(bufs, ref, dims))
+    tiles, launches = batch.tiles(G.ROUND_TRIP)
+    counts = {n: sum(1 for d in dims if d[0] == n) for n in BATCH_SIZES}
+    out = {"metric": "batched 256 mixed-size FP32 R2C+C2R throughput vs sequential plain-method launches",
+           "value": round(args.batch / (ms_batch * 1e-3), 1), "unit": "transforms/s (round trips)",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_batch, 5),
+           "higher_is_better": True, "dtype": "f32", "data": "synthetic uniform[-1,1) on device",
+           "config": {"workload": f"{args.batch} cubes from {BATCH_SIZES}, weights 1/N^3, seed 0",
+                      "counts": counts, "sum_S_bytes": s_bytes, "tiles": tiles, "launches": launches},
+           "plain_ms": round(ms_plain, 5), "batch_graph_ms": round(ms_batch, 5),
+           "batch_eager_ms": round(ms_batch_eager, 5),
+           "speedup_vs_plain": round(ms_plain / ms_batch, 3),
+           "gflops_batch": round(flops / (ms_batch * 1e-3) / 1e9, 1),
+           "alg_GBps_12S": round(12 * s_bytes / (ms_batch * 1e-3) / 1e9, 1),
+           "roundtrip_max_rel_err": rt_err}
+    batch.close()
+    for p in plans.values():
+        p.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=512, help="cube edge")
+    ap.add_argument("--shape", type=int, nargs=3, metavar=("NX", "NY", "NZ"), help="non-cubic grid")
+    ap.add_argument("--slab-share", type=int, default=0, metavar="P",
+                    help="time the local R2C half (z R2C + y C2C + pack) of rank 0's share at p = P")
+    ap.add_argument("--batch", type=int, default=0, metavar="COUNT",
+                    help="batched mode: COUNT mixed-size cube round trips as one graph launch")
     ap.add_argument("--prec", default="f32", choices=["f32", "f64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=3)
@@ -117,7 +259,11 @@ def main():
     import gpu_fft_amd as G
 
     n = args.n
-    nx = ny = nz = n
+    nx, ny, nz = args.shape if args.shape else (n, n, n)
+    if args.slab_share:
+        return slab_share(args, nx, ny, nz, args.slab_share)
+    if args.batch:
+        return batch_bench(args)
     K = nz // 2 + 1
     real_bytes = 4 if args.prec == "f32" else 8
     S = nx * ny * K * 2 * real_bytes
@@ -192,7 +338,7 @@ def main():
     dom = max(per_pass, key=lambda k: per_pass[k]["ms_mean"])
     dom_ms = per_pass[dom]["ms_mean"]
     achieved = 2 * S / (dom_ms * 1e-3) / 1e9
-    traffic = measured_traffic(dom, n, args.prec)
+    traffic = measured_traffic(dom, n, args.prec) if (nx, ny, nz) == (n, n, n) else None
 
     # sanity: the array must still hold the (normalised) input's magnitude
     rms = float(buf[:, :, :nz].float().pow(2).mean().sqrt())
@@ -215,7 +361,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if args.prec == "f32" else "f64",
         "data": "synthetic uniform[-1,1) generated on device (seed 0), pads zero",
-        "config": {"workload": f"{n}^3 {args.prec.upper()} in-place R2C+C2R round trip, p=1",
+        "config": {"workload": f"{nx}x{ny}x{nz} {args.prec.upper()} in-place R2C+C2R round trip, p=1",
                    "nx": nx, "ny": ny, "nz": nz, "procs": 1, "precision": args.prec,
                    "inverse_scale": "1/(Nx*Ny*Nz) fused"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
@@ -238,7 +384,7 @@ def main():
     if not args.no_cpu:
         workers = max(1, min(16, os.cpu_count() or 1))
         log(f"cpu baseline: {workers} threads")
-        out["cpu_baseline"] = cpu_baseline(n, args.cpu_reps, workers)
+        out["cpu_baseline"] = cpu_baseline((nx, ny, nz), args.cpu_reps, workers)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -260,3 +260,99 @@ def complex_view(buf, nx_out, ny_out, nz):
     """Complex (Nx, Ny/p, K) view of an in-place buffer (no copy)."""
     import torch
     return torch.view_as_complex(buf.view(-1, 2)).view(nx_out, ny_out, kdim(nz))
+
+
+def _bind_batch(L):
+    if getattr(L, "_gfft_batch_bound", False):
+        return L
+    i64, vp, sz = ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t
+    L.gfft_batch_query.restype = ctypes.c_int
+    L.gfft_batch_query.argtypes = [i64, i64, i64, ctypes.c_int]
+    L.gfft_batch_create.restype = ctypes.c_int
+    L.gfft_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(i64), ctypes.c_int]
+    L.gfft_batch_set_data.restype = ctypes.c_int
+    L.gfft_batch_set_data.argtypes = [vp, ctypes.POINTER(vp)]
+    L.gfft_batch_set_scales.restype = ctypes.c_int
+    L.gfft_batch_set_scales.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    for name in ("gfft_batch_exec", "gfft_batch_exec_graph"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [vp, ctypes.c_int, vp]
+    L.gfft_batch_data_bytes.restype = ctypes.c_int
+    L.gfft_batch_data_bytes.argtypes = [vp, ctypes.c_int, ctypes.POINTER(sz)]
+    L.gfft_batch_tiles.restype = ctypes.c_int
+    L.gfft_batch_tiles.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                   ctypes.POINTER(ctypes.c_uint32)]
+    L.gfft_batch_destroy.restype = ctypes.c_int
+    L.gfft_batch_destroy.argtypes = [vp]
+    L._gfft_batch_bound = True
+    return L
+
+
+R2C, C2R, ROUND_TRIP = 0, 1, 2
+
+
+def batch_supported(nx, ny, nz, precision="f32") -> bool:
+    return _bind_batch(lib()).gfft_batch_query(nx, ny, nz, _prec(precision)) == 0
+
+
+class Batch:
+    """Many independent in-place transforms of uneven sizes executed as one
+    unit, eagerly or as one hipGraph (gfft_batch_*).  ``dims`` is a list of
+    (nx, ny, nz); all edges of a transform are one supported size."""
+
+    def __init__(self, dims, precision="f32"):
+        L = _bind_batch(lib())
+        self.dims = [tuple(int(v) for v in d) for d in dims]
+        flat = (ctypes.c_int64 * (3 * len(self.dims)))(*[v for d in self.dims for v in d])
+        h = ctypes.c_void_p()
+        _check(L.gfft_batch_create(ctypes.byref(h), len(self.dims), flat, _prec(precision)), "gfft_batch_create")
+        self._h = h
+        self._keep = None
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("batch destroyed")
+        return self._h
+
+    def set_data(self, arrays) -> None:
+        ptrs = (ctypes.c_void_p * len(arrays))(*[a if isinstance(a, int) else a.data_ptr() for a in arrays])
+        _check(lib().gfft_batch_set_data(self.handle, ptrs), "gfft_batch_set_data")
+        self._keep = list(arrays)  # the device memory must outlive later launches
+
+    def set_scales(self, r2c=None, c2r=None) -> None:
+        n = len(self.dims)
+        a = (ctypes.c_double * n)(*r2c) if r2c is not None else None
+        b = (ctypes.c_double * n)(*c2r) if c2r is not None else None
+        _check(lib().gfft_batch_set_scales(self.handle, a, b), "gfft_batch_set_scales")
+
+    def exec(self, mode=ROUND_TRIP, stream=None) -> None:
+        _check(lib().gfft_batch_exec(self.handle, int(mode), _stream_handle(stream)), "gfft_batch_exec")
+
+    def exec_graph(self, mode=ROUND_TRIP, stream=None) -> None:
+        _check(lib().gfft_batch_exec_graph(self.handle, int(mode), _stream_handle(stream)), "gfft_batch_exec_graph")
+
+    def tiles(self, mode=ROUND_TRIP):
+        """(tiles, kernel launches) of one execution of ``mode``."""
+        t, n = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().gfft_batch_tiles(self.handle, int(mode), ctypes.byref(t), ctypes.byref(n)),
+               "gfft_batch_tiles")
+        return t.value, n.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib().gfft_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

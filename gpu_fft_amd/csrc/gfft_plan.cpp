@@ -534,6 +534,20 @@ int exec_c2r(const PlanCore& c, void* data, hipStream_t s) {
   return GFFT_OK;
 }
 
+std::vector<unsigned char> twiddle_table(const Schedule& s, int prec) {
+  std::vector<unsigned char> out;
+  if (prec == F32) {
+    const std::vector<float> t = make_twiddles<float>(s);
+    out.resize(t.size() * sizeof(float));
+    std::memcpy(out.data(), t.data(), out.size());
+  } else {
+    const std::vector<double> t = make_twiddles<double>(s);
+    out.resize(t.size() * sizeof(double));
+    std::memcpy(out.data(), t.data(), out.size());
+  }
+  return out;
+}
+
 // ---------------------------------------------------------------------------
 // The two halves of each direction around a slab exchange (any p, incl. 1).
 // ---------------------------------------------------------------------------

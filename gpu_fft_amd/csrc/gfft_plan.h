@@ -44,6 +44,9 @@ int exec_c2r(const PlanCore& c, void* data, hipStream_t s);
 int exec_axis(const PlanCore& c, int axis, bool inverse, void* data, hipStream_t s);
 int set_variant(PlanCore& c, int axis, int variant);
 size_t local_bytes(const PlanCore& c);
+int device_cu_count();
+// Stage twiddle table of a schedule (layout of gfft_device.h), as bytes.
+std::vector<unsigned char> twiddle_table(const Schedule& s, int prec);
 int exec_r2c_local(const PlanCore& c, void* data, void* sendbuf, hipStream_t s);
 int exec_r2c_finish(const PlanCore& c, const void* recvbuf, void* out, hipStream_t s);
 int exec_c2r_start(const PlanCore& c, const void* data, void* sendbuf, hipStream_t s);

@@ -38,6 +38,21 @@ int registry_f32(const KernelEntry** out) {
       GFFT_COLS(float, Sc256, 16, 16),
       GFFT_COLS(float, Sc256, 8, 8),
       GFFT_COLS(float, Sc768, 24, 8),
+      GFFT_COLS_PIPE(float, Sc1024, 16, 8, 2, 0),
+      GFFT_COLS_PIPE(float, Sc1024, 16, 8, 2, 1),
+      GFFT_COLS_PIPE(float, Sc1024, 16, 16, 1, 0),
+      GFFT_COLS_PIPE(float, Sc1024, 16, 16, 1, 1),
+      GFFT_COLS_PIPE(float, Sc768, 24, 8, 2, 0),
+      GFFT_COLS_PIPE(float, Sc768, 24, 8, 2, 1),
+      GFFT_COLS_PIPE(float, Sc768, 24, 16, 1, 0),
+      GFFT_COLS_PIPE(float, Sc768, 24, 16, 1, 1),
+      GFFT_COLS_PIPE(float, Sc256, 8, 16, 2, 0),
+      GFFT_COLS_PIPE(float, Sc256, 8, 16, 2, 1),
+      GFFT_COLS_PIPE(float, Sc512, 32, 32, 1, 0),
+      GFFT_COLS_PIPE(float, Sc512, 16, 32, 1, 0),
+      GFFT_ROWS_PIPE(float, Sc1024, 8, 2, 2, 1),
+      GFFT_ROWS_PIPE(float, Sc2048, 8, 1, 4, 1),
+      GFFT_ROWS_PIPE(float, Sc768, 24, 4, 2, 1),
       GFFT_COLS_PIPE(float, Sc512, 8, 16, 2, 0),
       GFFT_COLS_PIPE(float, Sc512, 16, 16, 2, 0),
       GFFT_COLS_PIPE(float, Sc512, 16, 16, 1, 0),

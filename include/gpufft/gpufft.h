@@ -163,6 +163,51 @@ int gfft_plan_describe(gfft_plan plan, char* buf, size_t len);
  * status gfft_plan_create would return for a geometry problem. */
 int gfft_query_plan(int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int prec, char* buf, size_t len);
 
+/* ---------------------------------------------------------------------------
+ * Batched mode: many independent in-place transforms of uneven sizes run as
+ * one unit of work, eagerly or as one hipGraph (no counterpart in the
+ * reference, which plans and runs one grid per process).  FP32; all edges of
+ * a transform are one size of {32, 48, 64, 96, 128, 160, 192, 256} (Nx and
+ * Ny may also be 1).  Each transform uses the layout of gfft_exec_r2c on its
+ * own array.  Transforms of one size share every launch: one launch per
+ * (size, axis pass) covers the tiles of all of them.
+ * ------------------------------------------------------------------------- */
+typedef struct gfft_batch_s* gfft_batch;
+
+/* GFFT_OK if (nx, ny, nz, prec) is supported by the batched kernels. */
+int gfft_batch_query(int64_t nx, int64_t ny, int64_t nz, int prec);
+
+/* dims: count triples (nx, ny, nz).  Allocates descriptors, tile tables,
+ * twiddles and internal streams. */
+int gfft_batch_create(gfft_batch* batch, int count, const int64_t* dims, int prec);
+
+/* Device pointers of the count in-place arrays (host array of pointers).
+ * Synchronises the device first (earlier launches read the old pointers). */
+int gfft_batch_set_data(gfft_batch batch, void* const* ptrs);
+
+/* Per-transform output scales (NULL = all 1.0), as gfft_plan_set_scale.
+ * Synchronises the device first. */
+int gfft_batch_set_scales(gfft_batch batch, const double* r2c_scale, const double* c2r_scale);
+
+/* mode 0 = R2C of every transform, 1 = C2R, 2 = R2C then C2R.  The launches
+ * of the different sizes run on internal streams forked from and joined
+ * back into `stream`: work queued on `stream` later starts after the batch. */
+int gfft_batch_exec(gfft_batch batch, int mode, gfft_stream_t stream);
+
+/* Same work as one hipGraph launch on `stream` (built on first use per
+ * mode, one branch per size).  The graph reads pointers and scales from
+ * device memory, so later gfft_batch_set_data / _set_scales calls take
+ * effect in replays. */
+int gfft_batch_exec_graph(gfft_batch batch, int mode, gfft_stream_t stream);
+
+int gfft_batch_data_bytes(gfft_batch batch, int index, size_t* bytes);
+
+/* Tiles one execution of `mode` processes, and (if launches is not NULL)
+ * the number of kernel launches it takes. */
+int gfft_batch_tiles(gfft_batch batch, int mode, uint32_t* tiles, uint32_t* launches);
+
+int gfft_batch_destroy(gfft_batch batch);
+
 #ifdef __cplusplus
 }
 #endif

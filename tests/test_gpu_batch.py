@@ -1,0 +1,161 @@
+"""Batched mode: many transforms of uneven sizes executed as one unit.
+Each result is checked against the FP64 oracle (FP32 tolerance 1e-6), is
+bitwise the same alone and in the mix and the same as the single-transform
+plan's, and graph replays (with eager work between them) reproduce the eager
+result."""
+import sys
+
+import numpy as np
+import pytest
+
+import gpu_fft_amd as G
+from oracle import gfft_oracle as O
+
+pytestmark = pytest.mark.gpu
+SIZES = [32, 48, 64, 96, 128, 160, 192, 256]
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def mix(count, seed, max_edge=256):
+    rng = np.random.default_rng(seed)
+    sizes = [s for s in SIZES if s <= max_edge]
+    w = np.array([1.0 / s ** 3 for s in sizes])
+    pick = rng.choice(sizes, size=count, p=w / w.sum())
+    dims = [(int(n), int(n), int(n)) for n in pick]
+    # a few degenerate axes too (length-1 x or y)
+    dims += [(1, 64, 64), (96, 1, 96), (1, 1, 160)]
+    return dims
+
+
+def inputs(torch, dims, seed):
+    out, host = [], []
+    for i, (nx, ny, nz) in enumerate(dims):
+        fp = O.pad_real(O.random_field(nx, ny, nz, seed=seed + i)).astype(np.float32)
+        host.append(fp)
+        out.append(torch.from_numpy(fp).to("cuda:0"))
+    return out, host
+
+
+def test_batch_r2c_then_c2r_matches_oracle():
+    import torch
+    dims = mix(40, seed=1, max_edge=128) + [(160, 160, 160), (192, 192, 192), (256, 256, 256)]
+    bufs, host = inputs(torch, dims, 100)
+    say(f"case batch {len(dims)} transforms")
+    with G.Batch(dims) as b:
+        b.set_data(bufs)
+        b.exec(G.R2C)
+        torch.cuda.synchronize()
+        for (nx, ny, nz), buf, fp in zip(dims, bufs, host):
+            got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
+            err = O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), nz))
+            assert err <= 1e-6, ((nx, ny, nz), err)
+        b.set_scales(c2r=[1.0 / (nx * ny * nz) for nx, ny, nz in dims])
+        b.exec(G.C2R)
+        torch.cuda.synchronize()
+        for (nx, ny, nz), buf, fp in zip(dims, bufs, host):
+            err = O.rel_l2(buf.cpu().numpy()[:, :, :nz], fp[:, :, :nz])
+            assert err <= 1e-6, ((nx, ny, nz), err)
+
+
+def test_alone_equals_in_mix_bitwise():
+    import torch
+    dims = mix(60, seed=2)
+    bufs, _ = inputs(torch, dims, 7)
+    ref = [b.clone() for b in bufs]
+    with G.Batch(dims) as b:
+        b.set_data(bufs)
+        b.exec(G.ROUND_TRIP)
+    torch.cuda.synchronize()
+    for i in (0, 7, len(dims) - 1, len(dims) // 2):
+        with G.Batch([dims[i]]) as one:
+            one.set_data([ref[i]])
+            one.exec(G.ROUND_TRIP)
+        torch.cuda.synchronize()
+        assert torch.equal(ref[i], bufs[i]), dims[i]
+
+
+def test_graph_replays_match_eager():
+    import torch
+    dims = mix(30, seed=3, max_edge=128)
+    bufs, host = inputs(torch, dims, 11)
+    eager = [x.clone() for x in bufs]
+    scales = [1.0 / (nx * ny * nz) for nx, ny, nz in dims]
+    with G.Batch(dims) as be:
+        be.set_data(eager)
+        be.set_scales(c2r=scales)
+        be.exec(G.ROUND_TRIP)
+    torch.cuda.synchronize()
+    with G.Batch(dims) as bg:
+        bg.set_scales(c2r=scales)
+        for rep in range(4):
+            work = [torch.from_numpy(h).to("cuda:0") for h in host]
+            bg.set_data(work)
+            bg.exec_graph(G.ROUND_TRIP)
+            torch.cuda.synchronize()
+            _ = torch.ones(1000, device="cuda:0").sum().item()  # eager work between replays
+            for w, e in zip(work, eager):
+                assert torch.equal(w, e), rep
+
+
+def test_batch_matches_single_plan_bitwise():
+    """The batched kernels and the single-transform plan (which may use other
+    radix schedules and tile shapes) agree to FP32 rounding."""
+    import torch
+    dims = [(64, 64, 64), (32, 32, 32), (256, 256, 256), (160, 160, 160)]
+    bufs, host = inputs(torch, dims, 21)
+    with G.Batch(dims) as b:
+        b.set_data(bufs)
+        b.exec(G.R2C)
+    torch.cuda.synchronize()
+    for (nx, ny, nz), buf, fp in zip(dims, bufs, host):
+        other = torch.from_numpy(fp).to("cuda:0")
+        with G.Plan(nx, ny, nz) as p:
+            p.r2c(other)
+        torch.cuda.synchronize()
+        err = O.rel_l2(buf.cpu().numpy(), other.cpu().numpy())
+        assert err <= 1e-6, ((nx, ny, nz), err)
+
+
+def test_eager_on_side_stream_orders_with_caller():
+    """exec on a non-default stream: work queued on that stream afterwards
+    sees the finished batch (fork/join of the internal streams)."""
+    import torch
+    dims = mix(12, seed=5, max_edge=128)
+    bufs, host = inputs(torch, dims, 31)
+    scales = [1.0 / (nx * ny * nz) for nx, ny, nz in dims]
+    side = torch.cuda.Stream()
+    with G.Batch(dims) as b:
+        b.set_data(bufs)
+        b.set_scales(c2r=scales)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(side):
+            b.exec(G.ROUND_TRIP, side)
+            sums = [x[..., :d[2]].double().sum() for x, d in zip(bufs, dims)]
+        side.synchronize()
+        for s, fp, (nx, ny, nz) in zip(sums, host, dims):
+            ref = float(fp[..., :nz].astype(np.float64).sum())
+            assert abs(float(s) - ref) <= 1e-4 * max(1.0, np.abs(fp).sum() / 1e3), ((nx, ny, nz), float(s), ref)
+
+
+def test_tile_and_launch_counts():
+    dims = [(64, 64, 64), (1, 64, 64), (32, 32, 32)]
+    with G.Batch(dims) as b:
+        tiles, launches = b.tiles(G.R2C)
+        # class 32: P=8 -> 4 threads/row -> 64 pairs or columns per tile
+        # class 64: P=8 -> 8 threads/row -> 32 pairs or columns per tile
+        z = -(-(32 * 32 // 2) // 64) + -(-(64 * 64 // 2) // 32) + -(-(64 // 2) // 32)
+        y = -(-(32 * 17) // 64) + -(-(64 * 33) // 32) + -(-(1 * 33) // 32)
+        x = -(-(32 * 17) // 64) + -(-(64 * 33) // 32)
+        assert tiles == z + y + x
+        assert launches == 6
+        assert b.tiles(G.ROUND_TRIP) == (2 * tiles, 12)
+
+
+def test_batch_rejects_unsupported_sizes():
+    with pytest.raises(G.GfftError):
+        G.Batch([(32, 32, 32), (40, 40, 40)])
+    with pytest.raises(G.GfftError):
+        G.Batch([(32, 64, 32)])
