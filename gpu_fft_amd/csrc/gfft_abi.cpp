@@ -43,6 +43,8 @@ const char* gfft_strerror(int status) {
     case GFFT_ERR_HIP: return "HIP runtime or kernel launch error";
     case GFFT_ERR_DEVICE_MISMATCH: return "current HIP device differs from the plan's device";
     case GFFT_ERR_BUFFER_TOO_SMALL: return "text buffer too small";
+    case GFFT_ERR_OVER_BUDGET: return "request larger than the server's whole HBM budget";
+    case GFFT_ERR_TIMEOUT: return "timed out waiting for the server to drain";
     default: return "unknown status";
   }
 }

@@ -1,0 +1,443 @@
+// Serving runtime: requests of mixed sizes / precisions / directions are
+// admitted in arrival order under an HBM budget and run on a few internal
+// streams; a poller thread stamps completions and admits the next requests.
+//
+// A request's life (host-staged):
+//   submit -> waiting FIFO -> admitted (budget + arena block) ->
+//   H2D copy, transform passes, D2H copy on one internal stream ->
+//   poller sees the last event -> block freed, result queued.
+// Device requests skip the copies and are transformed in place.
+//
+// Plans are built once per (Nx, Ny, Nz, prec) and shared by every request of
+// that geometry; exec only reads a plan, so the same plan runs on several
+// streams at once.  A request with its own output scale executes a shallow
+// copy of the plan with that scale (same kernels, same twiddle tables).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "gfft_arena.h"
+#include "gfft_plan.h"
+#include "gpufft/gpufft.h"
+
+// The ABI structs are bound field by field from other languages.
+static_assert(sizeof(gfft_request) == 80, "gfft_request layout");
+static_assert(sizeof(gfft_result) == 72, "gfft_result layout");
+static_assert(sizeof(gfft_server_stats_t) == 96, "gfft_server_stats_t layout");
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
+
+struct Events {
+  hipEvent_t e[4] = {};  // before copy-in, before transform, after transform, after copy-out
+};
+
+struct Req {
+  uint64_t id = 0;
+  gfft_request r{};
+  size_t bytes = 0;
+  bool device = false;
+  const gfft::PlanCore* plan = nullptr;
+  size_t off = gfft::Arena::kNone;
+  int stream = -1;
+  int ev = -1;  // index into the event pool
+  double t_adm = 0;
+};
+
+using Key = std::tuple<int64_t, int64_t, int64_t, int>;
+
+int request_bytes(int64_t nx, int64_t ny, int64_t nz, int prec, size_t* bytes) {
+  const int st = gfft::validate_dims(nx, ny, nz, 1, 0, prec);
+  if (st != GFFT_OK) return st;
+  const double b = (double)nx * (double)ny * (double)(2 * (nz / 2 + 1)) * (prec == GFFT_F32 ? 4.0 : 8.0);
+  if (b > 9.0e18) return GFFT_ERR_INVALID_DIMS;
+  *bytes = (size_t)nx * (size_t)ny * (size_t)(2 * (nz / 2 + 1)) * (prec == GFFT_F32 ? 4u : 8u);
+  return GFFT_OK;
+}
+
+}  // namespace
+
+struct gfft_server_s {
+  int device = 0;
+  int flags = 0;
+  char* arena_base = nullptr;
+  gfft::Arena arena;
+  gfft::Admission<Req*> adm;
+  std::vector<hipStream_t> streams;
+  std::vector<int> stream_load;
+  std::map<Key, std::unique_ptr<gfft::PlanCore>> plans;
+  size_t plan_bytes = 0;
+  std::vector<Events> ev_pool;
+  std::vector<int> ev_free;
+  std::vector<Req*> running;
+  std::vector<gfft_result> done;
+  size_t done_head = 0;
+  uint64_t next_id = 1, submitted = 0, completed = 0, rejected = 0;
+  std::mutex mu;
+  std::condition_variable cv_work;  // poller: something is running
+  std::condition_variable cv_idle;  // waiters: nothing waiting or running
+  std::thread poller;
+  bool stop = false;
+};
+
+namespace {
+
+int get_plan(gfft_server_s* s, int64_t nx, int64_t ny, int64_t nz, int prec, const gfft::PlanCore** out) {
+  const Key k{nx, ny, nz, prec};
+  auto it = s->plans.find(k);
+  if (it != s->plans.end()) {
+    *out = it->second.get();
+    return GFFT_OK;
+  }
+  std::unique_ptr<gfft::PlanCore> p(new (std::nothrow) gfft::PlanCore);
+  if (!p) return GFFT_ERR_OUT_OF_MEMORY;
+  int st = gfft::plan_host(*p, nx, ny, nz, 1, 0, prec);
+  if (st == GFFT_OK) st = gfft::upload_twiddles(*p);
+  if (st != GFFT_OK) {
+    gfft::free_twiddles(*p);
+    return st;
+  }
+  s->plan_bytes += p->z.tw_bytes + p->y.tw_bytes + p->x.tw_bytes;
+  *out = p.get();
+  s->plans.emplace(k, std::move(p));
+  return GFFT_OK;
+}
+
+int take_events(gfft_server_s* s) {
+  if (!s->ev_free.empty()) {
+    const int i = s->ev_free.back();
+    s->ev_free.pop_back();
+    return i;
+  }
+  Events e;
+  for (int j = 0; j < 4; ++j)
+    if (hipEventCreate(&e.e[j]) != hipSuccess) {
+      for (int q = 0; q < j; ++q) (void)hipEventDestroy(e.e[q]);
+      return -1;
+    }
+  s->ev_pool.push_back(e);
+  return (int)s->ev_pool.size() - 1;
+}
+
+hipEvent_t done_event(const gfft_server_s* s, const Req* q) { return s->ev_pool[q->ev].e[q->device ? 2 : 3]; }
+
+void finish(gfft_server_s* s, Req* q, int status, double t_done) {
+  gfft_result r{};
+  r.id = q->id;
+  r.tag = q->r.tag;
+  r.status = status;
+  r.stream = q->stream;
+  r.arrival = q->r.arrival;
+  r.admitted = q->t_adm;
+  r.done = t_done;
+  r.bytes = q->bytes;
+  if (q->ev >= 0 && status == GFFT_OK) {
+    const Events& e = s->ev_pool[q->ev];
+    float a = 0, b = 0;
+    if (hipEventElapsedTime(&a, e.e[1], e.e[2]) == hipSuccess) r.gpu_ms = a;
+    if (q->device)
+      r.staged_ms = r.gpu_ms;
+    else if (hipEventElapsedTime(&b, e.e[0], e.e[3]) == hipSuccess)
+      r.staged_ms = b;
+  }
+  if (q->ev >= 0) s->ev_free.push_back(q->ev);
+  if (q->off != gfft::Arena::kNone) s->arena.release(q->off, q->bytes);
+  if (q->stream >= 0) s->stream_load[q->stream]--;
+  if (q->t_adm > 0) s->adm.finished(q->bytes);
+  s->done.push_back(r);
+  ++s->completed;
+  delete q;
+}
+
+// Enqueue copy-in, passes, copy-out of an admitted request on its stream.
+int launch(gfft_server_s* s, Req* q) {
+  const hipStream_t st = s->streams[q->stream];
+  const Events& e = s->ev_pool[q->ev];
+  void* data = q->device ? q->r.data : s->arena_base + q->off;
+  if (!q->device) {
+    if (hipEventRecord(e.e[0], st) != hipSuccess) return GFFT_ERR_HIP;
+    if (hipMemcpyAsync(data, q->r.data, q->bytes, hipMemcpyHostToDevice, st) != hipSuccess) return GFFT_ERR_HIP;
+  }
+  if (hipEventRecord(e.e[1], st) != hipSuccess) return GFFT_ERR_HIP;
+  int rc;
+  const double sc = q->r.scale == 0.0 ? 1.0 : q->r.scale;
+  if (sc == 1.0 && q->plan->r2c_scale == 1.0 && q->plan->c2r_scale == 1.0) {
+    rc = q->r.direction == 0 ? gfft::exec_r2c(*q->plan, data, st) : gfft::exec_c2r(*q->plan, data, st);
+  } else {
+    gfft::PlanCore c = *q->plan;  // shares the plan's device tables
+    c.r2c_scale = q->r.direction == 0 ? sc : 1.0;
+    c.c2r_scale = q->r.direction == 1 ? sc : 1.0;
+    rc = q->r.direction == 0 ? gfft::exec_r2c(c, data, st) : gfft::exec_c2r(c, data, st);
+  }
+  if (rc != GFFT_OK) return rc;
+  if (hipEventRecord(e.e[2], st) != hipSuccess) return GFFT_ERR_HIP;
+  if (!q->device) {
+    if (hipMemcpyAsync(q->r.host_out, data, q->bytes, hipMemcpyDeviceToHost, st) != hipSuccess) return GFFT_ERR_HIP;
+    if (hipEventRecord(e.e[3], st) != hipSuccess) return GFFT_ERR_HIP;
+  }
+  return GFFT_OK;
+}
+
+// Admit from the head of the FIFO while the head fits.  Caller holds mu.
+void admit(gfft_server_s* s) {
+  bool started = false;
+  while (!s->adm.empty()) {
+    Req* q = s->adm.head();
+    if (!s->adm.fits(q->bytes)) break;
+    size_t off = gfft::Arena::kNone;
+    if (!q->device) {
+      off = s->arena.alloc(q->bytes);
+      if (off == gfft::Arena::kNone) break;
+    }
+    s->adm.pop_admitted(q->bytes);
+    q->off = off;
+    q->t_adm = now_s();
+    int st = q->plan ? GFFT_OK : get_plan(s, q->r.nx, q->r.ny, q->r.nz, q->r.prec, &q->plan);
+    if (st == GFFT_OK) {
+      q->ev = take_events(s);
+      if (q->ev < 0) st = GFFT_ERR_HIP;
+    }
+    if (st == GFFT_OK) {
+      int best = 0;
+      for (int i = 1; i < (int)s->streams.size(); ++i)
+        if (s->stream_load[i] < s->stream_load[best]) best = i;
+      q->stream = best;
+      s->stream_load[best]++;
+      st = launch(s, q);
+    }
+    if (st != GFFT_OK) {
+      // nothing of it may still be in flight when its block is reused
+      if (q->stream >= 0) (void)hipStreamSynchronize(s->streams[q->stream]);
+      finish(s, q, st, now_s());
+      continue;
+    }
+    s->running.push_back(q);
+    started = true;
+  }
+  if (started) s->cv_work.notify_one();
+  if (s->adm.empty() && s->running.empty()) s->cv_idle.notify_all();
+}
+
+void poll_loop(gfft_server_s* s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (!s->stop) {
+    if (s->running.empty()) {
+      s->cv_work.wait(lk, [&] { return s->stop || !s->running.empty(); });
+      continue;
+    }
+    bool any = false;
+    for (size_t i = 0; i < s->running.size();) {
+      Req* q = s->running[i];
+      const hipError_t e = hipEventQuery(done_event(s, q));
+      if (e == hipErrorNotReady) {
+        ++i;
+        continue;
+      }
+      const double t = now_s();
+      s->running[i] = s->running.back();
+      s->running.pop_back();
+      finish(s, q, e == hipSuccess ? GFFT_OK : GFFT_ERR_HIP, t);
+      any = true;
+    }
+    if (any) {
+      admit(s);
+      continue;
+    }
+    lk.unlock();
+    std::this_thread::yield();
+    lk.lock();
+  }
+}
+
+void teardown(gfft_server_s* s) {
+  if (s->poller.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->poller.join();
+  }
+  for (hipStream_t st : s->streams) (void)hipStreamSynchronize(st);
+  for (Req* q : s->running) delete q;
+  while (!s->adm.empty()) {
+    delete s->adm.head();
+    s->adm.pop_admitted(0);
+  }
+  for (auto& kv : s->plans) gfft::free_twiddles(*kv.second);
+  for (Events& e : s->ev_pool)
+    for (hipEvent_t x : e.e)
+      if (x) (void)hipEventDestroy(x);
+  for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+  if (s->arena_base) (void)hipFree(s->arena_base);
+}
+
+}  // namespace
+
+extern "C" {
+
+double gfft_server_clock(void) { return now_s(); }
+
+int gfft_request_bytes(int64_t nx, int64_t ny, int64_t nz, int prec, size_t* bytes) {
+  if (!bytes) return GFFT_ERR_INVALID_ARG;
+  return request_bytes(nx, ny, nz, prec, bytes);
+}
+
+int gfft_server_create(gfft_server* out, size_t budget_bytes, int nstreams, int flags) {
+  if (!out || budget_bytes == 0 || nstreams < 1 || nstreams > 64 || (flags & ~GFFT_SERVER_STAGING))
+    return GFFT_ERR_INVALID_ARG;
+  *out = nullptr;
+  gfft_server_s* s = new (std::nothrow) gfft_server_s;
+  if (!s) return GFFT_ERR_OUT_OF_MEMORY;
+  s->flags = flags;
+  s->adm = gfft::Admission<Req*>(budget_bytes);
+  int st = hipGetDevice(&s->device) == hipSuccess ? GFFT_OK : GFFT_ERR_HIP;
+  if (st == GFFT_OK && (flags & GFFT_SERVER_STAGING)) {
+    if (hipMalloc(&s->arena_base, budget_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      s->arena_base = nullptr;
+      st = GFFT_ERR_OUT_OF_MEMORY;
+    } else {
+      s->arena.reset(budget_bytes);
+    }
+  }
+  for (int i = 0; i < nstreams && st == GFFT_OK; ++i) {
+    hipStream_t h = nullptr;
+    if (hipStreamCreateWithFlags(&h, hipStreamNonBlocking) != hipSuccess) {
+      st = GFFT_ERR_HIP;
+      break;
+    }
+    s->streams.push_back(h);
+    s->stream_load.push_back(0);
+  }
+  if (st == GFFT_OK) {
+    try {
+      s->poller = std::thread(poll_loop, s);
+    } catch (...) {
+      st = GFFT_ERR_OUT_OF_MEMORY;
+    }
+  }
+  if (st != GFFT_OK) {
+    teardown(s);
+    delete s;
+    return st;
+  }
+  *out = s;
+  return GFFT_OK;
+}
+
+int gfft_server_prepare(gfft_server s, int64_t nx, int64_t ny, int64_t nz, int prec) {
+  if (!s) return GFFT_ERR_INVALID_ARG;
+  size_t b = 0;
+  int st = request_bytes(nx, ny, nz, prec, &b);
+  if (st != GFFT_OK) return st;
+  std::lock_guard<std::mutex> lk(s->mu);
+  const gfft::PlanCore* p = nullptr;
+  return get_plan(s, nx, ny, nz, prec, &p);
+}
+
+int gfft_server_submit(gfft_server s, const gfft_request* req, uint64_t* id) {
+  if (!s || !req || !req->data || (req->direction != 0 && req->direction != 1) || (req->flags & ~GFFT_REQ_DEVICE) ||
+      req->reserved != 0)
+    return GFFT_ERR_INVALID_ARG;
+  const bool device = (req->flags & GFFT_REQ_DEVICE) != 0;
+  if (!device && (!req->host_out || !s->arena_base)) return GFFT_ERR_INVALID_ARG;
+  size_t bytes = 0;
+  int st = request_bytes(req->nx, req->ny, req->nz, req->prec, &bytes);
+  if (st != GFFT_OK) return st;
+  if (bytes > s->adm.budget()) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    ++s->rejected;
+    return GFFT_ERR_OVER_BUDGET;
+  }
+  if (device) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, req->data) != hipSuccess || a.type != hipMemoryTypeDevice ||
+        a.device != s->device) {
+      (void)hipGetLastError();
+      return GFFT_ERR_INVALID_ARG;
+    }
+  }
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) return GFFT_ERR_HIP;
+  if (cur != s->device) return GFFT_ERR_DEVICE_MISMATCH;
+  Req* q = new (std::nothrow) Req;
+  if (!q) return GFFT_ERR_OUT_OF_MEMORY;
+  q->r = *req;
+  q->bytes = bytes;
+  q->device = device;
+  if (q->r.arrival < 0) q->r.arrival = now_s();
+  std::lock_guard<std::mutex> lk(s->mu);
+  q->id = s->next_id++;
+  ++s->submitted;
+  if (id) *id = q->id;
+  s->adm.push(q);
+  admit(s);
+  return GFFT_OK;
+}
+
+int gfft_server_poll(gfft_server s, gfft_result* out, int max, int* count) {
+  if (!s || !count || max < 0 || (max > 0 && !out)) return GFFT_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int n = 0;
+  while (n < max && s->done_head < s->done.size()) out[n++] = s->done[s->done_head++];
+  if (s->done_head == s->done.size()) {
+    s->done.clear();
+    s->done_head = 0;
+  }
+  *count = n;
+  return GFFT_OK;
+}
+
+int gfft_server_wait(gfft_server s, double timeout_s) {
+  if (!s) return GFFT_ERR_INVALID_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto idle = [&] { return s->adm.empty() && s->running.empty(); };
+  if (timeout_s <= 0) {
+    s->cv_idle.wait(lk, idle);
+    return GFFT_OK;
+  }
+  return s->cv_idle.wait_for(lk, std::chrono::duration<double>(timeout_s), idle) ? GFFT_OK : GFFT_ERR_TIMEOUT;
+}
+
+int gfft_server_stats(gfft_server s, gfft_server_stats_t* st) {
+  if (!s || !st) return GFFT_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::memset(st, 0, sizeof *st);
+  st->submitted = s->submitted;
+  st->completed = s->completed;
+  st->rejected = s->rejected;
+  st->waiting = s->adm.waiting();
+  st->running = s->running.size();
+  st->plans = s->plans.size();
+  st->budget = s->adm.budget();
+  st->in_flight_bytes = s->adm.in_flight();
+  st->peak_in_flight_bytes = s->adm.peak();
+  st->arena_bytes = s->arena.size();
+  st->arena_used = s->arena.used();
+  st->plan_bytes = s->plan_bytes;
+  return GFFT_OK;
+}
+
+int gfft_server_destroy(gfft_server s) {
+  if (!s) return GFFT_ERR_INVALID_ARG;
+  (void)gfft_server_wait(s, 0);
+  teardown(s);
+  delete s;
+  return GFFT_OK;
+}
+
+}  // extern "C"

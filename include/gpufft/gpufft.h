@@ -58,7 +58,9 @@ typedef enum {
   GFFT_ERR_OUT_OF_MEMORY = 6,     /* device allocation failed */
   GFFT_ERR_HIP = 7,               /* a HIP runtime call or kernel launch failed */
   GFFT_ERR_DEVICE_MISMATCH = 8,   /* executed with another current device than the plan's */
-  GFFT_ERR_BUFFER_TOO_SMALL = 9   /* text query buffer too small */
+  GFFT_ERR_BUFFER_TOO_SMALL = 9,  /* text query buffer too small */
+  GFFT_ERR_OVER_BUDGET = 10,      /* serving: one request needs more HBM than the whole budget */
+  GFFT_ERR_TIMEOUT = 11           /* serving: gfft_server_wait ran out of time */
 } gfft_status;
 
 /* Version number (GFFT_VERSION of the built library). */
@@ -207,6 +209,79 @@ int gfft_batch_data_bytes(gfft_batch batch, int index, size_t* bytes);
 int gfft_batch_tiles(gfft_batch batch, int mode, uint32_t* tiles, uint32_t* launches);
 
 int gfft_batch_destroy(gfft_batch batch);
+
+/* ---------------------------------------------------------------------------
+ * Serving runtime (no counterpart in the reference): R2C / C2R requests of
+ * mixed sizes and precisions arrive one by one and are executed as they are
+ * admitted.  Plans and twiddle tables stay resident per (Nx, Ny, Nz, prec).
+ * Admission is strict arrival order under an HBM byte budget: the head
+ * request starts once its array fits beside everything in flight (and, if
+ * host-staged, once the server's arena has a free block for it).  Admitted
+ * requests are spread over `nstreams` internal streams; a poller thread
+ * stamps completions.  Every entry point is thread-safe.
+ * ------------------------------------------------------------------------- */
+typedef struct gfft_server_s* gfft_server;
+
+#define GFFT_SERVER_STAGING 1 /* create: allocate a budget-sized arena for host-staged requests */
+#define GFFT_REQ_DEVICE 1     /* request: data is device memory, transformed in place */
+
+typedef struct {
+  int64_t nx, ny, nz;
+  int prec;       /* gfft_prec */
+  int direction;  /* 0 = R2C, 1 = C2R */
+  int flags;      /* GFFT_REQ_DEVICE, or 0 = host-staged through the arena */
+  int reserved;   /* 0 */
+  void* data;     /* device: the in-place array; host-staged: host input (pinned for overlap) */
+  void* host_out; /* host-staged: host output, may equal data; ignored for device requests */
+  double scale;   /* output scale fused into the last pass; 0 means 1 (unnormalised) */
+  double arrival; /* arrival time on gfft_server_clock(); < 0 = at submission */
+  uint64_t tag;   /* returned unchanged in the result */
+} gfft_request;
+
+typedef struct {
+  uint64_t id, tag;
+  int status;                     /* GFFT_OK, or why this request failed */
+  int stream;                     /* internal stream it ran on, -1 if it never ran */
+  double arrival, admitted, done; /* gfft_server_clock() seconds */
+  double gpu_ms;                  /* transform kernels only (device events) */
+  double staged_ms;               /* copy in + transform + copy out (device events); = gpu_ms for device */
+  size_t bytes;                   /* array bytes counted against the budget */
+} gfft_result;
+
+typedef struct {
+  uint64_t submitted, completed, rejected, waiting, running, plans;
+  size_t budget, in_flight_bytes, peak_in_flight_bytes, arena_bytes, arena_used, plan_bytes;
+} gfft_server_stats_t;
+
+/* Monotonic clock (seconds) used for arrival / admission / completion. */
+double gfft_server_clock(void);
+
+/* Bytes of a request's in-place array, Nx*Ny*2K reals (host-only). */
+int gfft_request_bytes(int64_t nx, int64_t ny, int64_t nz, int prec, size_t* bytes);
+
+/* budget_bytes bounds the arrays of admitted, unfinished requests; flags:
+ * GFFT_SERVER_STAGING.  Uses the current HIP device. */
+int gfft_server_create(gfft_server* srv, size_t budget_bytes, int nstreams, int flags);
+
+/* Build and keep the plan of a geometry now (otherwise built on first use,
+ * which allocates and synchronises). */
+int gfft_server_prepare(gfft_server srv, int64_t nx, int64_t ny, int64_t nz, int prec);
+
+/* Queue a request.  Invalid geometry, GFFT_ERR_OVER_BUDGET (the array alone
+ * exceeds the budget) and bad pointers are rejected here, nothing queued. */
+int gfft_server_submit(gfft_server srv, const gfft_request* req, uint64_t* id);
+
+/* Non-blocking: move up to max finished results (completion order) to out. */
+int gfft_server_poll(gfft_server srv, gfft_result* out, int max, int* count);
+
+/* Block until nothing is waiting or running; GFFT_ERR_TIMEOUT after timeout_s
+ * seconds (<= 0: no limit). */
+int gfft_server_wait(gfft_server srv, double timeout_s);
+
+int gfft_server_stats(gfft_server srv, gfft_server_stats_t* st);
+
+/* Waits for everything queued, then frees the server. */
+int gfft_server_destroy(gfft_server srv);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,158 @@
+"""Serving runtime on the GPU: served results are bitwise the same as the
+same request executed alone through the plan API, agree with the FP64
+oracle (FP32 rel L2 <= 1e-6, FP64 <= 1e-13), host-staged equals in-place
+device requests, admission keeps the in-flight bytes under the budget in
+strict arrival order, and bad requests are rejected before anything runs."""
+import sys
+
+import numpy as np
+import pytest
+
+import gpu_fft_amd as G
+from gpu_fft_amd import serving as S
+from oracle import gfft_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_input(torch, nx, ny, nz, prec, direction, seed, device="cuda:0"):
+    """R2C: padded random real field; C2R: the R2C spectrum of one (Hermitian)."""
+    fp = O.pad_real(O.random_field(nx, ny, nz, seed))
+    if direction == S.C2R:
+        spec = O.r2c_p1(fp, nz)
+        fp = np.ascontiguousarray(np.stack([spec.real, spec.imag], axis=-1).reshape(nx, ny, -1))
+    dt = np.float32 if prec == "f32" else np.float64
+    return torch.from_numpy(fp.astype(dt)).to(device)
+
+
+def mix(count, seed, max_edge):
+    rng = np.random.default_rng(seed)
+    sizes = [n for n in S.SERVE_SIZES if n <= max_edge]
+    out = []
+    for i in range(count):
+        n = int(rng.choice(sizes))
+        out.append(((n, n, n), "f32" if rng.random() < 0.8 else "f64", int(rng.integers(2))))
+    # the odd sizes and non-cubic grids are served too
+    out += [((75, 75, 75), "f64", S.C2R), ((81, 64, 75), "f32", S.R2C), ((64, 80, 81), "f64", S.R2C)]
+    return out
+
+
+def alone(torch, dims, prec, direction, x):
+    y = x.clone()
+    with G.Plan(*dims, precision=prec) as p:
+        (p.r2c if direction == S.R2C else p.c2r)(y)
+    torch.cuda.synchronize()
+    return y
+
+
+def test_served_mix_equals_alone_bitwise_and_oracle():
+    import torch
+    reqs = mix(36, seed=1, max_edge=128)
+    say(f"case served mix of {len(reqs)} device requests")
+    inputs = [make_input(torch, *d, p, dr, seed=10 + i) for i, (d, p, dr) in enumerate(reqs)]
+    work = [x.clone() for x in inputs]
+    with S.Server(budget_bytes=8 << 30, nstreams=4) as srv:
+        ids = [srv.submit(d, p, dr, w, tag=i) for i, ((d, p, dr), w) in enumerate(zip(reqs, work))]
+        res = srv.drain(timeout_s=120)
+        st = srv.stats()
+    assert len(res) == len(reqs) and all(r["status"] == 0 for r in res)
+    assert sorted(r["tag"] for r in res) == list(range(len(reqs)))
+    assert st["completed"] == len(reqs) and st["in_flight_bytes"] == 0
+    assert len(set(ids)) == len(ids)
+    for i, ((d, p, dr), x, w) in enumerate(zip(reqs, inputs, work)):
+        ref = alone(torch, d, p, dr, x)
+        assert torch.equal(w, ref), (i, d, p, dr)
+        nx, ny, nz = d
+        tol = 1e-6 if p == "f32" else 1e-13
+        if dr == S.R2C:
+            want = O.r2c_p1(x.cpu().numpy().astype(np.float64), nz)
+            got = G.complex_view(w, nx, ny, nz).cpu().numpy()
+        else:
+            xc = x.cpu().numpy().astype(np.float64).reshape(nx, ny, -1, 2)
+            want = O.c2r_p1(xc[..., 0] + 1j * xc[..., 1], nz)
+            got = w.cpu().numpy()[..., :nz]
+        err = O.rel_l2(got, want)
+        assert err <= tol, (i, d, p, dr, err)
+
+
+def test_host_staged_equals_device():
+    import torch
+    reqs = mix(10, seed=2, max_edge=96)
+    say(f"case host-staged vs device, {len(reqs)} requests")
+    dev_in = [make_input(torch, *d, p, dr, seed=40 + i) for i, (d, p, dr) in enumerate(reqs)]
+    host_in = [x.cpu().pin_memory() for x in dev_in]
+    host_out = [torch.empty_like(h).pin_memory() for h in host_in]
+    with S.Server(budget_bytes=1 << 30, nstreams=3, staging=True) as srv:
+        for (d, p, dr), hi, ho in zip(reqs, host_in, host_out):
+            srv.submit(d, p, dr, hi, host_out=ho, device=False)
+        for (d, p, dr), x in zip(reqs, dev_in):
+            srv.submit(d, p, dr, x)
+        res = srv.drain(timeout_s=120)
+    assert all(r["status"] == 0 for r in res) and len(res) == 2 * len(reqs)
+    for x, ho in zip(dev_in, host_out):
+        assert torch.equal(x.cpu(), ho)
+    staged = [r for r in res if r["staged_ms"] >= r["gpu_ms"] > 0]
+    assert len(staged) == len(res)
+
+
+def test_admission_respects_budget_in_arrival_order():
+    import torch
+    dims, prec = (256, 256, 256), "f32"
+    nbytes = S.request_bytes(*dims, prec)
+    budget = 3 * nbytes + nbytes // 2  # room for three at a time
+    base = make_input(torch, *dims, prec, S.R2C, seed=70)
+    bufs = [base.clone() for _ in range(12)]
+    say("case admission budget, 12 requests of 256^3 f32")
+    with S.Server(budget_bytes=budget, nstreams=4) as srv:
+        for i, b in enumerate(bufs):
+            srv.submit(dims, prec, S.R2C, b, tag=i)
+        res = srv.drain(timeout_s=120)
+        st = srv.stats()
+    assert all(r["status"] == 0 for r in res)
+    assert st["peak_in_flight_bytes"] <= budget
+    assert st["peak_in_flight_bytes"] >= 2 * nbytes  # several ran side by side
+    by_tag = sorted(res, key=lambda r: r["tag"])
+    adm = [r["admitted"] for r in by_tag]
+    assert adm == sorted(adm)  # strict FIFO admission
+    for r in res:
+        assert r["arrival"] <= r["admitted"] <= r["done"]
+
+
+def test_over_budget_and_bad_requests_rejected():
+    import torch
+    x = make_input(torch, 64, 64, 64, "f32", S.R2C, seed=3)
+    with S.Server(budget_bytes=1 << 20, nstreams=1) as srv:
+        with pytest.raises(G.GfftError) as e:
+            srv.submit((64, 64, 64), "f32", S.R2C, x)  # 1.1 MB > 1 MiB budget
+        assert e.value.status == 10
+        with pytest.raises(G.GfftError) as e:
+            srv.submit((64, 64, 0), "f32", S.R2C, x)
+        assert e.value.status == 2
+        host = torch.zeros(8, 8, 10)
+        with pytest.raises(G.GfftError) as e:
+            srv.submit((8, 8, 8), "f32", S.R2C, host, device=True)  # not device memory
+        assert e.value.status == 1
+        with pytest.raises(G.GfftError) as e:  # host-staged needs a staging server
+            srv.submit((8, 8, 8), "f32", S.R2C, host, device=False)
+        assert e.value.status == 1
+        st = srv.stats()
+    assert st["rejected"] == 1 and st["submitted"] == 0
+
+
+def test_request_scale_normalises_round_trip():
+    import torch
+    dims = (96, 80, 75)
+    x = make_input(torch, *dims, "f64", S.R2C, seed=5)
+    y = x.clone()
+    with S.Server(budget_bytes=1 << 30, nstreams=2) as srv:
+        srv.submit(dims, "f64", S.R2C, y)
+        srv.wait(60)
+        srv.submit(dims, "f64", S.C2R, y, scale=1.0 / (96 * 80 * 75))
+        res = srv.drain(timeout_s=60)
+    assert all(r["status"] == 0 for r in res)
+    err = O.rel_l2(y.cpu().numpy()[..., :75], x.cpu().numpy()[..., :75])
+    assert err <= 1e-13, err
