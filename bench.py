@@ -228,7 +228,149 @@ def batch_bench(args):
     batch.close()
     for p in plans.values():
         p.close()
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def serve_dims(count: int, seed: int):
+    """Config-5 request mix: edge log-uniform in [64, 512] rounded to the
+    nearest 2^a 3^b 5^c size, 80% FP32 / 20% FP64, R2C or C2R with equal
+    probability."""
+    import numpy as np
+    from gpu_fft_amd.serving import SERVE_SIZES
+    rng = np.random.default_rng(seed)
+    sizes = np.array(SERVE_SIZES)
+    edges = np.exp(rng.uniform(math.log(64), math.log(512), size=count))
+    pick = sizes[np.abs(sizes[None, :] - edges[:, None]).argmin(axis=1)]
+    prec = np.where(rng.random(count) < 0.8, "f32", "f64")
+    dirs = rng.integers(0, 2, size=count)
+    return [(int(n), str(p), int(d)) for n, p, d in zip(pick, prec, dirs)]
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(math.ceil(q / 100.0 * len(xs))) - 1))
+    return xs[k]
+
+
+def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: float = 64.0, nstreams: int = 4,
+                seed: int = 0) -> dict:
+    """Poisson-arrival serving run of the config-5 mix.
+
+    1. Plans of all 84 (size, precision) geometries are built up front
+       (plan / twiddle residency; untimed).
+    2. Capacity: a closed burst of requests submitted at once; capacity =
+       requests / (last completion - first arrival).
+    3. Open loop: arrivals at rate load * capacity (exponential gaps); each
+       request is submitted at its scheduled arrival time, latency = its
+       completion - that arrival.
+    Device requests are transformed in place in one resident array per
+    geometry (requests of one geometry share it; each request carries the
+    unitary scale 1/sqrt(N) so repeated transforms keep the values bounded).
+    Host-staged requests copy from / to one pinned host array per precision
+    through the server's budget-sized arena."""
+    import torch
+    from gpu_fft_amd import serving as SV
+    dev = torch.device("cuda:0")
+    reqs_cap = serve_dims(max(64, n_requests // 2), seed + 1)
+    reqs = serve_dims(n_requests, seed)
+    keys = sorted({(n, p) for n, p, _ in reqs + reqs_cap})
+    srv = SV.Server(int(budget_gb * (1 << 30)), nstreams=nstreams, staging=staged)
+    for n, p in keys:
+        srv.prepare(n, n, n, p)
+    bufs, hin, hout = {}, {}, {}
+    if staged:
+        for p in ("f32", "f64"):
+            mx = max((SV.request_bytes(n, n, n, p) for n, q in keys if q == p), default=0)
+            if mx:
+                dt = torch.float32 if p == "f32" else torch.float64
+                esz = 4 if p == "f32" else 8
+                hin[p] = (torch.rand(mx // esz, dtype=dt) * 2 - 1).pin_memory()
+                hout[p] = torch.empty(mx // esz, dtype=dt).pin_memory()
+    else:
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        for n, p in keys:
+            dt = torch.float32 if p == "f32" else torch.float64
+            bufs[(n, p)] = torch.rand((n, n, 2 * (n // 2 + 1)), dtype=dt, device=dev, generator=gen) * 2 - 1
+        torch.cuda.synchronize()
+
+    def submit(n, p, d, arrival, tag=0):
+        scale = 1.0 / math.sqrt(n ** 3)
+        if staged:
+            return srv.submit((n, n, n), p, d, hin[p], host_out=hout[p], device=False, scale=scale,
+                              arrival=arrival, tag=tag)
+        return srv.submit((n, n, n), p, d, bufs[(n, p)], scale=scale, arrival=arrival, tag=tag)
+
+    # warm every plan once (first launches load code objects)
+    for n, p in keys:
+        submit(n, p, 0, -1.0)
+    srv.drain(timeout_s=300)
+
+    # capacity: closed burst
+    t0 = SV.clock()
+    for n, p, d in reqs_cap:
+        submit(n, p, d, t0)
+    res = srv.drain(timeout_s=300)
+    span = max(r["done"] for r in res) - t0
+    capacity = len(reqs_cap) / span
+    rate = load * capacity
+
+    # open loop, Poisson arrivals
+    import numpy as np
+    gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=len(reqs))
+    sched = SV.clock() + 0.01 + np.cumsum(gaps)
+    late = 0.0
+    for i, ((n, p, d), ta) in enumerate(zip(reqs, sched)):
+        while SV.clock() < ta:
+            pass
+        late = max(late, SV.clock() - ta)
+        submit(n, p, d, float(ta), tag=i)
+    res = srv.drain(timeout_s=600)
+    st = srv.stats()
+    srv.close()
+    ok = [r for r in res if r["status"] == 0]
+    lat = [(r["done"] - r["arrival"]) * 1e3 for r in ok]
+    queue = [(r["admitted"] - r["arrival"]) * 1e3 for r in ok]
+    span = max(r["done"] for r in ok) - float(sched[0])
+    bytes_total = sum(r["bytes"] for r in ok)
+    flops = sum(5.0 * n ** 3 * math.log2(n ** 3) for n, _, _ in reqs)
+    by_prec = {p: [(r["done"] - r["arrival"]) * 1e3 for r in ok if reqs[r["tag"]][1] == p] for p in ("f32", "f64")}
+    return {
+        "mode": "host-staged (H2D + transform + D2H)" if staged else "device-resident, in place",
+        "requests": len(reqs), "failed": len(res) - len(ok),
+        "capacity_req_per_s": round(capacity, 1), "offered_req_per_s": round(rate, 1), "load": load,
+        "throughput_req_per_s": round(len(ok) / span, 1),
+        "throughput_GFLOPs": round(flops / span / 1e9, 1),
+        "throughput_GBps_6S": round(6 * bytes_total / span / 1e9, 1) if not staged else None,
+        "latency_ms": {"p50": round(pct(lat, 50), 4), "p99": round(pct(lat, 99), 4), "mean": round(sum(lat) / len(lat), 4),
+                       "max": round(max(lat), 4)},
+        "latency_ms_f32": {"p50": round(pct(by_prec["f32"], 50), 4), "p99": round(pct(by_prec["f32"], 99), 4)}
+        if by_prec["f32"] else None,
+        "latency_ms_f64": {"p50": round(pct(by_prec["f64"], 50), 4), "p99": round(pct(by_prec["f64"], 99), 4)}
+        if by_prec["f64"] else None,
+        "queue_ms": {"p50": round(pct(queue, 50), 4), "p99": round(pct(queue, 99), 4)},
+        "gpu_ms_mean": round(sum(r["gpu_ms"] for r in ok) / len(ok), 4),
+        "staged_ms_mean": round(sum(r["staged_ms"] for r in ok) / len(ok), 4),
+        "budget_bytes": st["budget"], "peak_in_flight_bytes": st["peak_in_flight_bytes"],
+        "plans_resident": st["plans"], "plan_bytes": st["plan_bytes"], "streams": nstreams,
+        "max_submit_lateness_ms": round(late * 1e3, 4),
+        "workload": "edges log-uniform in [64,512] -> nearest of 42 sizes 2^a3^b5^c, 80/20 FP32/FP64, "
+                    "R2C/C2R 50/50, Poisson arrivals, seed 0",
+    }
+
+
+def serve_main(args) -> dict:
+    dev_res = serve_bench(args.serve, staged=False, load=args.serve_load)
+    host_res = serve_bench(max(100, args.serve // 5), staged=True, load=args.serve_load)
+    return {"metric": "served mixed-size R2C/C2R requests: p50/p99 arrival-to-completion latency at "
+                      f"{args.serve_load:.0%} of capacity",
+            "value": dev_res["latency_ms"]["p99"], "unit": "ms (p99 latency, device-resident requests)",
+            "n_gpus": 1, "steps": args.serve, "warmup": 0, "higher_is_better": False,
+            "dtype": "f32/f64 (80/20)", "data": "synthetic, resident per geometry",
+            "config": {"workload": dev_res["workload"], "budget_GB": 64},
+            "device": dev_res, "host_staged": host_res}
 
 
 def main():
@@ -242,6 +384,11 @@ def main():
                     help="time the local R2C half (z R2C + y C2C + pack) of rank 0's share at p = P")
     ap.add_argument("--batch", type=int, default=0, metavar="COUNT",
                     help="batched mode: COUNT mixed-size cube round trips as one graph launch")
+    ap.add_argument("--serve", type=int, default=0, metavar="REQUESTS",
+                    help="serving mode: REQUESTS Poisson arrivals of the mixed-size request stream")
+    ap.add_argument("--serve-load", type=float, default=0.8, help="offered load as a fraction of capacity")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default run without the batched and served sub-results")
     ap.add_argument("--prec", default="f32", choices=["f32", "f64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=3)
@@ -263,7 +410,11 @@ def main():
     if args.slab_share:
         return slab_share(args, nx, ny, nz, args.slab_share)
     if args.batch:
-        return batch_bench(args)
+        print(json.dumps(batch_bench(args)), flush=True)
+        return
+    if args.serve:
+        print(json.dumps(serve_main(args)), flush=True)
+        return
     K = nz // 2 + 1
     real_bytes = 4 if args.prec == "f32" else 8
     S = nx * ny * K * 2 * real_bytes
@@ -387,6 +538,16 @@ def main():
         out["cpu_baseline"] = cpu_baseline((nx, ny, nz), args.cpu_reps, workers)
     else:
         out["cpu_baseline"] = None
+    if not args.no_extras:
+        log("batched mode: 256 mixed cubes")
+        b = batch_bench(argparse.Namespace(batch=256, steps=20, warmup=3))
+        out["batch"] = {k: b[k] for k in ("plain_ms", "batch_graph_ms", "batch_eager_ms", "speedup_vs_plain",
+                                          "gflops_batch", "roundtrip_max_rel_err")}
+        out["batch"]["workload"] = b["config"]["workload"]
+        log("serving: device-resident requests")
+        out["serving"] = serve_bench(1500, staged=False, load=0.8)
+        log("serving: host-staged requests")
+        out["serving_host_staged"] = serve_bench(300, staged=True, load=0.8)
     print(json.dumps(out), flush=True)
 
 

@@ -251,9 +251,13 @@ __device__ __forceinline__ void stage_barrier() {
 }
 
 // ---------------------------------------------------------------------------
-// Register-resident engine.  Thread t (0 <= t < TPT = N/P) owns butterflies
-// i = t + TPT*q (q < P/R) of every stage; v[q*R + r] is input/output r of
-// butterfly q.  After run(), v[q*R + r] holds X[t + TPT*q + r*N/R_last].
+// Register-resident engine.  TPT = N/P threads share one transform.  At
+// stage s (radix R) thread t owns butterflies i = t + TPT*q for q < NB(s) =
+// ceil((N/R) / TPT); when TPT*NB(s) exceeds N/R the surplus slots are idle
+// (guarded), so any radix order works with any P dividing N.  v[q*R + r] is
+// input/output r of butterfly q; PV register slots cover the widest stage.
+// When every radix divides P, NB(s) = P/R, PV = P and no guard is emitted.
+// After run(), v[q*RL + r] holds X[t + TPT*q + r*N/RL] (valid slots).
 // LB selects LDS-only barriers between stages (see lds_barrier).
 // ---------------------------------------------------------------------------
 template <typename T, bool INV, typename SC, int P, bool LB = false>
@@ -263,13 +267,34 @@ struct RegEngine {
   static constexpr int TPT = N / P;
   static_assert(N % P == 0, "P must divide N");
 
+  static constexpr int NB(int s) { return (N / SC::R[s] + TPT - 1) / TPT; }
+  static constexpr bool EXACT(int s) { return NB(s) * TPT == N / SC::R[s]; }
+  static constexpr int pv() {
+    int m = 0;
+    for (int s = 0; s < SC::S; ++s) m = NB(s) * SC::R[s] > m ? NB(s) * SC::R[s] : m;
+    return m;
+  }
+  static constexpr int PV = pv();
+  static constexpr int R0 = SC::R[0];
+  static constexpr int RL = SC::R[SC::S - 1];
+  static constexpr int NB0 = NB(0);
+  static constexpr int NBL = NB(SC::S - 1);
+
   template <int s>
-  __device__ __forceinline__ static void bfly(C (&v)[P], int t, const C* __restrict__ tw) {
+  __device__ __forceinline__ static bool live(int t, int q) {
+    return EXACT(s) || t + TPT * q < N / SC::R[s];
+  }
+  // slot (t, q) of the first / last stage holds a real butterfly
+  __device__ __forceinline__ static bool live0(int t, int q) { return live<0>(t, q); }
+  __device__ __forceinline__ static bool liveL(int t, int q) { return live<SC::S - 1>(t, q); }
+
+  template <int s>
+  __device__ __forceinline__ static void bfly(C (&v)[PV], int t, const C* __restrict__ tw) {
     constexpr int R = SC::R[s];
     constexpr int p = SC::p(s);
-    static_assert(P % R == 0, "every radix must divide P");
 #pragma unroll
-    for (int q = 0; q < P / R; ++q) {
+    for (int q = 0; q < NB(s); ++q) {
+      if (!live<s>(t, q)) continue;
       if constexpr (p > 1) {
         const int i = t + TPT * q;
         const int k = i % p;
@@ -282,11 +307,12 @@ struct RegEngine {
   }
 
   template <int s, typename L>
-  __device__ __forceinline__ static void put(C (&v)[P], int t, L& lds) {
+  __device__ __forceinline__ static void put(C (&v)[PV], int t, L& lds) {
     constexpr int R = SC::R[s];
     constexpr int p = SC::p(s);
 #pragma unroll
-    for (int q = 0; q < P / R; ++q) {
+    for (int q = 0; q < NB(s); ++q) {
+      if (!live<s>(t, q)) continue;
       const int i = t + TPT * q;
       const int k = i % p;
       const int j = (i - k) * R + k;
@@ -296,10 +322,11 @@ struct RegEngine {
   }
 
   template <int s, typename L>
-  __device__ __forceinline__ static void get(C (&v)[P], int t, L& lds) {
+  __device__ __forceinline__ static void get(C (&v)[PV], int t, L& lds) {
     constexpr int R = SC::R[s];
 #pragma unroll
-    for (int q = 0; q < P / R; ++q) {
+    for (int q = 0; q < NB(s); ++q) {
+      if (!live<s>(t, q)) continue;
       const int i = t + TPT * q;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q * R + r] = lds.at(i + r * (N / R));
@@ -307,7 +334,7 @@ struct RegEngine {
   }
 
   template <int s, typename L>
-  __device__ __forceinline__ static void run_from(C (&v)[P], int t, L& lds, const C* __restrict__ tw) {
+  __device__ __forceinline__ static void run_from(C (&v)[PV], int t, L& lds, const C* __restrict__ tw) {
     bfly<s>(v, t, tw);
     if constexpr (s + 1 < SC::S) {
       stage_barrier<LB>();
@@ -318,21 +345,19 @@ struct RegEngine {
     }
   }
 
-  // v holds stage-0 inputs: v[q*R0 + r] = x[t + TPT*q + r*N/R0]
+  // v holds stage-0 inputs: v[q*R0 + r] = x[t + TPT*q + r*N/R0] (live slots)
   template <typename L>
-  __device__ __forceinline__ static void run(C (&v)[P], int t, L& lds, const C* __restrict__ tw) {
+  __device__ __forceinline__ static void run(C (&v)[PV], int t, L& lds, const C* __restrict__ tw) {
     run_from<0>(v, t, lds, tw);
   }
 
   // element index of stage-0 input / last-stage output slot (q, r)
   __device__ __forceinline__ static constexpr int in_index(int t, int q, int r) {
-    return t + TPT * q + r * (N / SC::R[0]);
+    return t + TPT * q + r * (N / R0);
   }
   __device__ __forceinline__ static constexpr int out_index(int t, int q, int r) {
-    return t + TPT * q + r * (N / SC::R[SC::S - 1]);
+    return t + TPT * q + r * (N / RL);
   }
-  static constexpr int R0 = SC::R[0];
-  static constexpr int RL = SC::R[SC::S - 1];
 };
 
 // ---------------------------------------------------------------------------

@@ -85,9 +85,13 @@ struct KernelEntry {
   int order;        // persistent tile order: 0 contiguous ranges, 1 round robin
 };
 
-// Registry of compiled variants; returns the number of entries.
+// Registry units of compiled variants; each returns its number of entries.
 int registry_f32(const KernelEntry** out);
 int registry_f64(const KernelEntry** out);
+int registry_smooth_f32_a(const KernelEntry** out);
+int registry_smooth_f32_b(const KernelEntry** out);
+int registry_smooth_f64_a(const KernelEntry** out);
+int registry_smooth_f64_b(const KernelEntry** out);
 
 // Run-time-schedule (generic) launchers.
 struct RtSchedHost {

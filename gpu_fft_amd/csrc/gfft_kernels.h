@@ -50,34 +50,39 @@ struct RowsR2C {
   static constexpr int N = SC::N;
   static constexpr int TPT = E::TPT;
   static constexpr int NT = G * TPT;
+  static_assert(NT <= 1024, "workgroup larger than 1024 threads");
   static constexpr int KC = N / 2 + 1;  // half-spectrum bins per row (= RowGeom::K)
   static constexpr int LROW = RowPad<N>::lrow(KC);
 
   // stage-0 inputs of pair `pair` (rows 2*pair and 2*pair + 1): z = a + i b
-  __device__ __forceinline__ static void load(C (&v)[P], const T* __restrict__ data, const RowGeom& g, int64_t pair,
-                                              int t) {
+  __device__ __forceinline__ static void load(C (&v)[E::PV], const T* __restrict__ data, const RowGeom& g,
+                                              int64_t pair, int t) {
     const int64_t ra = 2 * pair, rb = ra + 1;
     const bool va = ra < g.nrows, vb = rb < g.nrows;
     const T* A = data + ra * 2 * (int64_t)KC;
     const T* B = A + 2 * (int64_t)KC;
 #pragma unroll
-    for (int q = 0; q < P / E::R0; ++q)
+    for (int q = 0; q < E::NB0; ++q) {
+      if (!E::live0(t, q)) continue;
 #pragma unroll
       for (int r = 0; r < E::R0; ++r) {
         const int n = E::in_index(t, q, r);
         v[q * E::R0 + r] = mk<T>(va ? ld_stream(A + n) : (T)0, vb ? ld_stream(B + n) : (T)0);
       }
+    }
   }
 
   // after E::run: Z goes through LDS, is split into the two half spectra and
   // stored over the pair's own rows
-  __device__ __forceinline__ static void finish(C (&v)[P], T* __restrict__ data, const RowGeom& g, int64_t pair,
-                                                int t, RowLds<T, N>& L) {
+  __device__ __forceinline__ static void finish(C (&v)[E::PV], T* __restrict__ data, const RowGeom& g,
+                                                int64_t pair, int t, RowLds<T, N>& L) {
     stage_barrier<LB>();
 #pragma unroll
-    for (int q = 0; q < P / E::RL; ++q)
+    for (int q = 0; q < E::NBL; ++q) {
+      if (!E::liveL(t, q)) continue;
 #pragma unroll
       for (int r = 0; r < E::RL; ++r) L.at(E::out_index(t, q, r)) = v[q * E::RL + r];
+    }
     stage_barrier<LB>();
     const int64_t ra = 2 * pair, rb = ra + 1;
     const bool va = ra < g.nrows, vb = rb < g.nrows;
@@ -102,7 +107,7 @@ struct RowsR2C {
                                               int64_t pair0, C* smem) {
     const int gi = threadIdx.x / TPT, t = threadIdx.x % TPT;
     RowLds<T, N> L{smem + gi * LROW};
-    C v[P];
+    C v[E::PV];
     load(v, data, g, pair0 + gi, t);
     E::run(v, t, L, tw);
     finish(v, data, g, pair0 + gi, t, L);
@@ -119,6 +124,7 @@ struct RowsC2R {
   static constexpr int N = SC::N;
   static constexpr int TPT = E::TPT;
   static constexpr int NT = G * TPT;
+  static_assert(NT <= 1024, "workgroup larger than 1024 threads");
   static constexpr int KC = N / 2 + 1;
   static constexpr int NI = (KC + TPT - 1) / TPT;  // half-spectrum bins per thread and row
   static constexpr int LROW = RowPad<N>::lrow(KC);
@@ -152,7 +158,7 @@ struct RowsC2R {
   }
 
   // LDS staging of the spectra, then the stage-0 inputs of the complex FFT
-  __device__ __forceinline__ static void build(C (&v)[P], const C (&xa)[NI], const C (&xb)[NI], C* S, int t) {
+  __device__ __forceinline__ static void build(C (&v)[E::PV], const C (&xa)[NI], const C (&xb)[NI], C* S, int t) {
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
       const int k = t + i * TPT;
@@ -163,38 +169,39 @@ struct RowsC2R {
     }
     stage_barrier<LB>();
 #pragma unroll
-    for (int q = 0; q < P / E::R0; ++q)
+    for (int q = 0; q < E::NB0; ++q) {
+      if (!E::live0(t, q)) continue;
 #pragma unroll
       for (int r = 0; r < E::R0; ++r) v[q * E::R0 + r] = hermitian_z(S, E::in_index(t, q, r));
+    }
   }
 
-  __device__ __forceinline__ static void store(C (&v)[P], T* __restrict__ data, const RowGeom& g, int64_t pair,
-                                               int t) {
+  __device__ __forceinline__ static void store(C (&v)[E::PV], T* __restrict__ data, const RowGeom& g,
+                                               int64_t pair, int t) {
     const int64_t ra = 2 * pair, rb = ra + 1;
     const bool va = ra < g.nrows, vb = rb < g.nrows;
     T* A = data + ra * 2 * (int64_t)KC;
     T* B = A + 2 * (int64_t)KC;
-    if (g.scale != 1.0) {
-      const T sc = (T)g.scale;
+    const T sc = (T)g.scale;
 #pragma unroll
-      for (int i = 0; i < P; ++i) v[i] = mk<T>(v[i].x * sc, v[i].y * sc);
-    }
-#pragma unroll
-    for (int q = 0; q < P / E::RL; ++q)
+    for (int q = 0; q < E::NBL; ++q) {
+      if (!E::liveL(t, q)) continue;
 #pragma unroll
       for (int r = 0; r < E::RL; ++r) {
         const int n = E::out_index(t, q, r);
-        const C z = v[q * E::RL + r];
+        C z = v[q * E::RL + r];
+        if (g.scale != 1.0) z = mk<T>(z.x * sc, z.y * sc);
         if (va) st_stream(A + n, z.x);
         if (vb) st_stream(B + n, z.y);
       }
+    }
   }
 
   __device__ __forceinline__ static void tile(T* __restrict__ data, RowGeom g, const C* __restrict__ tw,
                                               int64_t pair0, C* smem) {
     const int gi = threadIdx.x / TPT, t = threadIdx.x % TPT;
     C* S = smem + gi * LROW;
-    C xa[NI], xb[NI], v[P];
+    C xa[NI], xb[NI], v[E::PV];
     load(xa, xb, data, g, pair0 + gi, t);
     build(v, xa, xb, S, t);
     RowLds<T, N> L{S};
@@ -224,40 +231,45 @@ struct ColsPass {
   static constexpr int N = SC::N;
   static constexpr int TPT = E::TPT;
   static constexpr int NT = CW * TPT;
+  static_assert(NT <= 1024, "workgroup larger than 1024 threads");
 
   // `in` and `out` may be the same array (in-place pass): no __restrict__
-  __device__ __forceinline__ static void load(C (&v)[P], const C* in, const ColGeom& g, uint32_t tile,
+  __device__ __forceinline__ static void load(C (&v)[E::PV], const C* in, const ColGeom& g, uint32_t tile,
                                               int c, int t) {
     const uint32_t col = tile * CW + c;
     const bool valid = col < g.ncols;
     const C* p = in + (valid ? map_base(g.in, col) : 0);
 #pragma unroll
-    for (int q = 0; q < P / E::R0; ++q)
+    for (int q = 0; q < E::NB0; ++q) {
+      if (!E::live0(t, q)) continue;
 #pragma unroll
       for (int r = 0; r < E::R0; ++r)
         v[q * E::R0 + r] = valid ? ld_stream(p + map_row<N>(g.in, E::in_index(t, q, r))) : mk<T>(0, 0);
+    }
   }
 
-  __device__ __forceinline__ static void store(C (&v)[P], C* out, const ColGeom& g, uint32_t tile, int c,
+  __device__ __forceinline__ static void store(C (&v)[E::PV], C* out, const ColGeom& g, uint32_t tile, int c,
                                                int t) {
     const uint32_t col = tile * CW + c;
     if (col >= g.ncols) return;
     C* p = out + map_base(g.out, col);
-    if (g.scale != 1.0) {
-      const T sc = (T)g.scale;
+    const T sc = (T)g.scale;
 #pragma unroll
-      for (int i = 0; i < P; ++i) v[i] = mk<T>(v[i].x * sc, v[i].y * sc);
+    for (int q = 0; q < E::NBL; ++q) {
+      if (!E::liveL(t, q)) continue;
+#pragma unroll
+      for (int r = 0; r < E::RL; ++r) {
+        C z = v[q * E::RL + r];
+        if (g.scale != 1.0) z = mk<T>(z.x * sc, z.y * sc);
+        st_stream(p + map_row<N>(g.out, E::out_index(t, q, r)), z);
+      }
     }
-#pragma unroll
-    for (int q = 0; q < P / E::RL; ++q)
-#pragma unroll
-      for (int r = 0; r < E::RL; ++r) st_stream(p + map_row<N>(g.out, E::out_index(t, q, r)), v[q * E::RL + r]);
   }
 
   __device__ __forceinline__ static void tile(const C* in, C* out, const ColGeom& g, const C* __restrict__ tw,
                                               uint32_t tile_id, C* smem) {
     const int c = threadIdx.x % CW, t = threadIdx.x / CW;
-    C v[P];
+    C v[E::PV];
     load(v, in, g, tile_id, c, t);
     ColLds<T, CW> L{smem, c};
     E::run(v, t, L, tw);
@@ -347,11 +359,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPipe<T, INV, SC, P,
   copy_twiddles<SC>(stw, tw, K_::NT);
   const int c = threadIdx.x % CW, t = threadIdx.x / CW;
   ColLds<T, CW> L{smem, c};
-  cx<T> cur[P], nxt[P];
+  cx<T> cur[K_::E::PV], nxt[K_::E::PV];
   K_::load(nxt, in, g, walk.at(0), c, t);
   for (uint32_t k = 0; k < walk.count; ++k) {
 #pragma unroll
-    for (int i = 0; i < P; ++i) cur[i] = nxt[i];
+    for (int i = 0; i < K_::E::PV; ++i) cur[i] = nxt[i];
     if (k + 1 < walk.count) K_::load(nxt, in, g, walk.at(k + 1), c, t);
     // the first stage uses no twiddles; two barriers precede the first
     // twiddle read, so the LDS copy above is complete by then
@@ -372,11 +384,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsR2C<T, SC, P, G, tr
   copy_twiddles<SC>(stw, tw, K_::NT);
   const int gi = threadIdx.x / K_::TPT, t = threadIdx.x % K_::TPT;
   RowLds<T, SC::N> L{smem + gi * K_::LROW};
-  cx<T> cur[P], nxt[P];
+  cx<T> cur[K_::E::PV], nxt[K_::E::PV];
   K_::load(nxt, data, g, (int64_t)walk.at(0) * G + gi, t);
   for (uint32_t k = 0; k < walk.count; ++k) {
 #pragma unroll
-    for (int i = 0; i < P; ++i) cur[i] = nxt[i];
+    for (int i = 0; i < K_::E::PV; ++i) cur[i] = nxt[i];
     if (k + 1 < walk.count) K_::load(nxt, data, g, (int64_t)walk.at(k + 1) * G + gi, t);
     K_::E::run(cur, t, L, stw);
     K_::finish(cur, data, g, (int64_t)walk.at(k) * G + gi, t, L);
@@ -395,7 +407,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsC2R<T, SC, P, G, tr
   const int gi = threadIdx.x / K_::TPT, t = threadIdx.x % K_::TPT;
   cx<T>* S = smem + gi * K_::LROW;
   RowLds<T, SC::N> L{S};
-  cx<T> ca[K_::NI], cb[K_::NI], na[K_::NI], nb[K_::NI], v[P];
+  cx<T> ca[K_::NI], cb[K_::NI], na[K_::NI], nb[K_::NI], v[K_::E::PV];
   K_::load(na, nb, data, g, (int64_t)walk.at(0) * G + gi, t);
   for (uint32_t k = 0; k < walk.count; ++k) {
 #pragma unroll

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <string>
@@ -55,7 +56,29 @@ bool factor_235(int64_t n, std::vector<int>& radices) {
   return true;
 }
 
-int registry(int prec, const KernelEntry** tab) { return prec == F32 ? registry_f32(tab) : registry_f64(tab); }
+// All compiled variants of a precision, in registration order (units in the
+// order below, so a unit's first entry for a size is that size's default).
+const std::vector<KernelEntry>& all_entries(int prec) {
+  using Unit = int (*)(const KernelEntry**);
+  auto gather = [](std::initializer_list<Unit> units) {
+    std::vector<KernelEntry> v;
+    for (Unit u : units) {
+      const KernelEntry* t = nullptr;
+      const int n = u(&t);
+      v.insert(v.end(), t, t + n);
+    }
+    return v;
+  };
+  static const std::vector<KernelEntry> f32 = gather({registry_f32, registry_smooth_f32_a, registry_smooth_f32_b});
+  static const std::vector<KernelEntry> f64 = gather({registry_f64, registry_smooth_f64_a, registry_smooth_f64_b});
+  return prec == F32 ? f32 : f64;
+}
+
+int registry(int prec, const KernelEntry** tab) {
+  const std::vector<KernelEntry>& v = all_entries(prec);
+  *tab = v.data();
+  return (int)v.size();
+}
 
 bool same_sched(const Schedule& a, const Schedule& b) {
   if (a.N != b.N || a.S != b.S) return false;

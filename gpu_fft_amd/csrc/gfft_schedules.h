@@ -20,6 +20,43 @@ using Sc768 = Sched<8, 8, 4, 3>;
 using Sc1024 = Sched<8, 8, 4, 4>;
 using Sc2048 = Sched<8, 8, 8, 4>;
 
+// 2^a 3^b 5^c sizes of the serving range [64, 512] without a power-of-two
+// schedule above: 8s first (a 2^4 factor as 4 x 4), then 5s, then 3s.
+using Sc72 = Sched<8, 3, 3>;
+using Sc75 = Sched<5, 5, 3>;
+using Sc80 = Sched<4, 4, 5>;
+using Sc81 = Sched<3, 3, 3, 3>;
+using Sc90 = Sched<2, 5, 3, 3>;
+using Sc100 = Sched<4, 5, 5>;
+using Sc108 = Sched<4, 3, 3, 3>;
+using Sc120 = Sched<8, 5, 3>;
+using Sc125 = Sched<5, 5, 5>;
+using Sc135 = Sched<5, 3, 3, 3>;
+using Sc144 = Sched<4, 4, 3, 3>;
+using Sc150 = Sched<2, 5, 5, 3>;
+using Sc162 = Sched<2, 3, 3, 3, 3>;
+using Sc180 = Sched<4, 5, 3, 3>;
+using Sc200 = Sched<8, 5, 5>;
+using Sc216 = Sched<8, 3, 3, 3>;
+using Sc225 = Sched<5, 5, 3, 3>;
+using Sc240 = Sched<4, 4, 5, 3>;
+using Sc243 = Sched<3, 3, 3, 3, 3>;
+using Sc250 = Sched<2, 5, 5, 5>;
+using Sc270 = Sched<2, 5, 3, 3, 3>;
+using Sc288 = Sched<8, 4, 3, 3>;
+using Sc300 = Sched<4, 5, 5, 3>;
+using Sc320 = Sched<8, 8, 5>;
+using Sc324 = Sched<4, 3, 3, 3, 3>;
+using Sc360 = Sched<8, 5, 3, 3>;
+using Sc375 = Sched<5, 5, 5, 3>;
+using Sc400 = Sched<4, 4, 5, 5>;
+using Sc405 = Sched<5, 3, 3, 3, 3>;
+using Sc432 = Sched<4, 4, 3, 3, 3>;
+using Sc450 = Sched<2, 5, 5, 3, 3>;
+using Sc480 = Sched<8, 4, 5, 3>;
+using Sc486 = Sched<2, 3, 3, 3, 3, 3>;
+using Sc500 = Sched<4, 5, 5, 5>;
+
 // Default variant of a size: z R2C, z C2R, strided forward, strided inverse.
 // The first variant registered for a (precision, kind, N) is the default;
 // later ones are tuning alternatives (gfft_plan_tune / gfft_plan_set_variant).

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -52,8 +53,17 @@ struct Req {
   const gfft::PlanCore* plan = nullptr;
   size_t off = gfft::Arena::kNone;
   int stream = -1;
-  int ev = -1;  // index into the event pool
+  int ev = -1;     // index into the event pool
+  Events e;        // copy of the pool entry's handles (the pool may grow meanwhile)
   double t_adm = 0;
+  hipEvent_t done_event() const { return e.e[device ? 2 : 3]; }
+};
+
+// Requests on one stream finish in submission order: only the oldest one
+// needs to be checked.
+struct Lane {
+  std::deque<Req*> q;
+  size_t bytes = 0;  // array bytes of the lane's unfinished requests
 };
 
 using Key = std::tuple<int64_t, int64_t, int64_t, int>;
@@ -76,12 +86,12 @@ struct gfft_server_s {
   gfft::Arena arena;
   gfft::Admission<Req*> adm;
   std::vector<hipStream_t> streams;
-  std::vector<int> stream_load;
+  std::vector<Lane> lanes;  // one per stream
+  size_t n_running = 0;
   std::map<Key, std::unique_ptr<gfft::PlanCore>> plans;
   size_t plan_bytes = 0;
   std::vector<Events> ev_pool;
   std::vector<int> ev_free;
-  std::vector<Req*> running;
   std::vector<gfft_result> done;
   size_t done_head = 0;
   uint64_t next_id = 1, submitted = 0, completed = 0, rejected = 0;
@@ -131,8 +141,6 @@ int take_events(gfft_server_s* s) {
   return (int)s->ev_pool.size() - 1;
 }
 
-hipEvent_t done_event(const gfft_server_s* s, const Req* q) { return s->ev_pool[q->ev].e[q->device ? 2 : 3]; }
-
 void finish(gfft_server_s* s, Req* q, int status, double t_done) {
   gfft_result r{};
   r.id = q->id;
@@ -144,7 +152,7 @@ void finish(gfft_server_s* s, Req* q, int status, double t_done) {
   r.done = t_done;
   r.bytes = q->bytes;
   if (q->ev >= 0 && status == GFFT_OK) {
-    const Events& e = s->ev_pool[q->ev];
+    const Events& e = q->e;
     float a = 0, b = 0;
     if (hipEventElapsedTime(&a, e.e[1], e.e[2]) == hipSuccess) r.gpu_ms = a;
     if (q->device)
@@ -154,7 +162,7 @@ void finish(gfft_server_s* s, Req* q, int status, double t_done) {
   }
   if (q->ev >= 0) s->ev_free.push_back(q->ev);
   if (q->off != gfft::Arena::kNone) s->arena.release(q->off, q->bytes);
-  if (q->stream >= 0) s->stream_load[q->stream]--;
+  if (q->stream >= 0) s->lanes[q->stream].bytes -= q->bytes;
   if (q->t_adm > 0) s->adm.finished(q->bytes);
   s->done.push_back(r);
   ++s->completed;
@@ -164,7 +172,7 @@ void finish(gfft_server_s* s, Req* q, int status, double t_done) {
 // Enqueue copy-in, passes, copy-out of an admitted request on its stream.
 int launch(gfft_server_s* s, Req* q) {
   const hipStream_t st = s->streams[q->stream];
-  const Events& e = s->ev_pool[q->ev];
+  const Events& e = q->e;
   void* data = q->device ? q->r.data : s->arena_base + q->off;
   if (!q->device) {
     if (hipEventRecord(e.e[0], st) != hipSuccess) return GFFT_ERR_HIP;
@@ -207,14 +215,18 @@ void admit(gfft_server_s* s) {
     int st = q->plan ? GFFT_OK : get_plan(s, q->r.nx, q->r.ny, q->r.nz, q->r.prec, &q->plan);
     if (st == GFFT_OK) {
       q->ev = take_events(s);
-      if (q->ev < 0) st = GFFT_ERR_HIP;
+      if (q->ev < 0)
+        st = GFFT_ERR_HIP;
+      else
+        q->e = s->ev_pool[q->ev];
     }
     if (st == GFFT_OK) {
+      // the stream with the least outstanding work (array bytes)
       int best = 0;
-      for (int i = 1; i < (int)s->streams.size(); ++i)
-        if (s->stream_load[i] < s->stream_load[best]) best = i;
+      for (int i = 1; i < (int)s->lanes.size(); ++i)
+        if (s->lanes[i].bytes < s->lanes[best].bytes) best = i;
       q->stream = best;
-      s->stream_load[best]++;
+      s->lanes[best].bytes += q->bytes;
       st = launch(s, q);
     }
     if (st != GFFT_OK) {
@@ -223,42 +235,45 @@ void admit(gfft_server_s* s) {
       finish(s, q, st, now_s());
       continue;
     }
-    s->running.push_back(q);
+    s->lanes[q->stream].q.push_back(q);
+    ++s->n_running;
     started = true;
   }
   if (started) s->cv_work.notify_one();
-  if (s->adm.empty() && s->running.empty()) s->cv_idle.notify_all();
+  if (s->adm.empty() && s->n_running == 0) s->cv_idle.notify_all();
 }
 
+// The lock is not held while events are queried: only this thread removes
+// requests from the lanes, so the oldest request of a lane stays valid.
 void poll_loop(gfft_server_s* s) {
   (void)hipSetDevice(s->device);
-  std::unique_lock<std::mutex> lk(s->mu);
-  while (!s->stop) {
-    if (s->running.empty()) {
-      s->cv_work.wait(lk, [&] { return s->stop || !s->running.empty(); });
-      continue;
+  std::vector<std::pair<int, Req*>> fronts;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] { return s->stop || s->n_running > 0; });
+      if (s->stop) return;
+      fronts.clear();
+      for (int i = 0; i < (int)s->lanes.size(); ++i)
+        if (!s->lanes[i].q.empty()) fronts.emplace_back(i, s->lanes[i].q.front());
     }
     bool any = false;
-    for (size_t i = 0; i < s->running.size();) {
-      Req* q = s->running[i];
-      const hipError_t e = hipEventQuery(done_event(s, q));
-      if (e == hipErrorNotReady) {
-        ++i;
-        continue;
-      }
+    for (const auto& f : fronts) {
+      const hipError_t e = hipEventQuery(f.second->done_event());
+      if (e == hipErrorNotReady) continue;
       const double t = now_s();
-      s->running[i] = s->running.back();
-      s->running.pop_back();
-      finish(s, q, e == hipSuccess ? GFFT_OK : GFFT_ERR_HIP, t);
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->lanes[f.first].q.pop_front();
+      --s->n_running;
+      finish(s, f.second, e == hipSuccess ? GFFT_OK : GFFT_ERR_HIP, t);
       any = true;
     }
     if (any) {
+      std::lock_guard<std::mutex> lk(s->mu);
       admit(s);
-      continue;
+    } else {
+      std::this_thread::yield();
     }
-    lk.unlock();
-    std::this_thread::yield();
-    lk.lock();
   }
 }
 
@@ -272,7 +287,8 @@ void teardown(gfft_server_s* s) {
     s->poller.join();
   }
   for (hipStream_t st : s->streams) (void)hipStreamSynchronize(st);
-  for (Req* q : s->running) delete q;
+  for (Lane& l : s->lanes)
+    for (Req* q : l.q) delete q;
   while (!s->adm.empty()) {
     delete s->adm.head();
     s->adm.pop_admitted(0);
@@ -321,7 +337,7 @@ int gfft_server_create(gfft_server* out, size_t budget_bytes, int nstreams, int 
       break;
     }
     s->streams.push_back(h);
-    s->stream_load.push_back(0);
+    s->lanes.emplace_back();
   }
   if (st == GFFT_OK) {
     try {
@@ -405,7 +421,7 @@ int gfft_server_poll(gfft_server s, gfft_result* out, int max, int* count) {
 int gfft_server_wait(gfft_server s, double timeout_s) {
   if (!s) return GFFT_ERR_INVALID_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
-  auto idle = [&] { return s->adm.empty() && s->running.empty(); };
+  auto idle = [&] { return s->adm.empty() && s->n_running == 0; };
   if (timeout_s <= 0) {
     s->cv_idle.wait(lk, idle);
     return GFFT_OK;
@@ -421,7 +437,7 @@ int gfft_server_stats(gfft_server s, gfft_server_stats_t* st) {
   st->completed = s->completed;
   st->rejected = s->rejected;
   st->waiting = s->adm.waiting();
-  st->running = s->running.size();
+  st->running = s->n_running;
   st->plans = s->plans.size();
   st->budget = s->adm.budget();
   st->in_flight_bytes = s->adm.in_flight();

@@ -2,6 +2,7 @@
 include/gpufft/gpufft.h declares, and its host-side planner validates
 geometries and picks kernels as documented."""
 import ctypes
+import math
 import os
 import re
 
@@ -28,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_strerror_covers_codes():
-    for code in range(0, 10):
+    for code in range(0, 12):
         s = G.lib().gfft_strerror(code).decode()
         assert s and s != "unknown status"
     assert G.lib().gfft_strerror(99).decode() == "unknown status"
@@ -78,10 +79,29 @@ def test_plan_choice_512_uses_register_kernels():
 
 
 def test_plan_choice_generic_and_identity():
-    d = G.query_plan(1, 75, 243, precision="f64")
+    d = G.query_plan(1, 45, 27, precision="f64")
     assert d["x"]["pass"] == "identity"
-    assert d["y"]["engine"] == "generic" and sorted(d["y"]["radices"]) == [3, 5, 5]
-    assert d["z"]["engine"] == "generic" and d["z"]["radices"] == [3, 3, 3, 3, 3]
+    assert d["y"]["engine"] == "generic" and sorted(d["y"]["radices"]) == [3, 3, 5]
+    assert d["z"]["engine"] == "generic" and d["z"]["radices"] == [3, 3, 3]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_every_serving_size_has_compiled_kernels(prec):
+    # 2^a 3^b 5^c edges in [64, 512]: all three passes use the register engine
+    sizes = [n for n in range(64, 513) if _smooth(n)]
+    assert len(sizes) == 42
+    for n in sizes:
+        d = G.query_plan(n, n, n, precision=prec)
+        for ax in ("z", "y", "x"):
+            assert d[ax]["engine"] == "register", (n, prec, ax)
+            assert math.prod(d[ax]["radices"]) == n
+
+
+def _smooth(n):
+    for p in (2, 3, 5):
+        while n % p == 0:
+            n //= p
+    return n == 1
 
 
 def test_twiddle_table_sizes():
