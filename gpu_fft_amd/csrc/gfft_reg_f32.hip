@@ -12,7 +12,7 @@ int registry_f32(const KernelEntry** out) {
       GFFT_SIZE(float, Sc64, 8, 32, 8, 16),
       GFFT_SIZE(float, Sc96, 24, 32, 24, 16),
       GFFT_SIZE(float, Sc128, 8, 16, 8, 16),
-      GFFT_SIZE(float, Sc160, 40, 32, 40, 16),
+      GFFT_SIZE(float, Sc160, 10, 16, 10, 16),  // P=10: 16 threads per transform
       GFFT_SIZE(float, Sc192, 24, 16, 24, 16),
       GFFT_SIZE(float, Sc256, 8, 8, 8, 16),
       GFFT_SIZE(float, Sc384, 24, 8, 24, 16),
@@ -25,6 +25,9 @@ int registry_f32(const KernelEntry** out) {
       GFFT_SIZE(float, Sc1024, 8, 2, 16, 16),
       GFFT_SIZE(float, Sc2048, 8, 1, 16, 8),
       // tuning alternatives
+      GFFT_SIZE(float, Sc160, 40, 32, 40, 16),
+      GFFT_SIZE(float, Sc160, 20, 32, 20, 32),
+      GFFT_SIZE(float, Sc160, 16, 24, 16, 16),
       GFFT_COLS(float, Sc512, 16, 16),
       GFFT_COLS(float, Sc512, 8, 8),
       GFFT_COLS(float, Sc512, 16, 8),

@@ -12,7 +12,7 @@ int registry_f64(const KernelEntry** out) {
       GFFT_SIZE(double, Sc64, 8, 32, 8, 8),
       GFFT_SIZE(double, Sc96, 24, 16, 24, 8),
       GFFT_SIZE(double, Sc128, 8, 16, 8, 8),
-      GFFT_SIZE(double, Sc160, 40, 16, 40, 8),
+      GFFT_SIZE(double, Sc160, 10, 16, 10, 16),  // P=10: 16 threads per transform
       GFFT_SIZE(double, Sc192, 24, 8, 24, 8),
       GFFT_SIZE(double, Sc256, 8, 8, 8, 8),
       GFFT_SIZE(double, Sc384, 24, 4, 24, 8),
@@ -24,6 +24,10 @@ int registry_f64(const KernelEntry** out) {
       GFFT_SIZE(double, Sc1024, 8, 2, 16, 8),
       GFFT_SIZE(double, Sc2048, 8, 1, 16, 4),
       // tuning alternatives
+      GFFT_SIZE(double, Sc160, 40, 16, 40, 8),
+      GFFT_SIZE(double, Sc160, 20, 16, 20, 16),
+      GFFT_SIZE(double, Sc160, 16, 24, 16, 16),
+      GFFT_COLS(double, Sc160, 40, 16),
       GFFT_COLS(double, Sc512, 8, 4),
       GFFT_COLS(double, Sc512, 16, 4),
       GFFT_ROWS(double, Sc512, 8, 2),
