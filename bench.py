@@ -84,15 +84,8 @@ def measured_traffic(pass_name: str, n: int, prec: str):
         return None
     with open(path) as fh:
         doc = json.load(fh)
-    fam = {"z_r2c": "k_rows_r2c", "z_c2r": "k_rows_c2r"}.get(pass_name, "k_cols")
-    inv = pass_name.endswith("_inv")
-    for name, v in doc["kernels"].items():
-        if not name.startswith(fam):
-            continue
-        if fam == "k_cols" and (", true," in name) != inv:
-            continue
-        return int(v["read_bytes"] + v["write_bytes"])
-    return None
+    v = doc.get("passes", {}).get(pass_name)
+    return int(v["read_bytes"] + v["write_bytes"]) if v else None
 
 
 def slab_share(args, nx, ny, nz, procs):
