@@ -76,6 +76,8 @@ def lib() -> ctypes.CDLL:
     L.gfft_plan_set_variant.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.gfft_plan_set_chunking.restype = ctypes.c_int
     L.gfft_plan_set_chunking.argtypes = [vp, ctypes.c_int64]
+    L.gfft_plan_set_walk.restype = ctypes.c_int
+    L.gfft_plan_set_walk.argtypes = [vp, ctypes.c_int]
     L.gfft_plan_set_scale.restype = ctypes.c_int
     L.gfft_plan_set_scale.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.gfft_exec_axis.restype = ctypes.c_int
@@ -198,6 +200,11 @@ class Plan:
     def set_chunking(self, planes: int) -> None:
         """Alternate z and y passes per block of ``planes`` x-planes (0 = off)."""
         _check(lib().gfft_plan_set_chunking(self.handle, int(planes)), "gfft_plan_set_chunking")
+
+    def set_walk(self, mode: int) -> None:
+        """0: every pass walks its tiles in order; 1: passes alternate the
+        order so each starts where the previous one ended (cache reuse)."""
+        _check(lib().gfft_plan_set_walk(self.handle, int(mode)), "gfft_plan_set_walk")
 
     def set_scale(self, r2c_scale: float = 1.0, c2r_scale: float = 1.0) -> None:
         """Output scales fused into the last pass of each direction."""

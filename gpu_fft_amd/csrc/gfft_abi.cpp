@@ -149,6 +149,12 @@ int gfft_plan_set_chunking(gfft_plan plan, int64_t planes) {
   return GFFT_OK;
 }
 
+int gfft_plan_set_walk(gfft_plan plan, int mode) {
+  if (!plan || mode < 0 || mode > 1) return GFFT_ERR_INVALID_ARG;
+  plan->core.walk = mode;
+  return GFFT_OK;
+}
+
 int gfft_plan_set_scale(gfft_plan plan, double r2c_scale, double c2r_scale) {
   if (!plan) return GFFT_ERR_INVALID_ARG;
   plan->core.r2c_scale = r2c_scale;

@@ -91,7 +91,8 @@ __global__ __launch_bounds__(kGenThreads) void k_gen_rows_r2c(T* data, RowGeom g
   cx<T>* buf0 = reinterpret_cast<cx<T>*>(raw);
   const int N = sc.N, K = g.K;
   cx<T>* buf1 = buf0 + G * N;
-  const int64_t pair0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * G;
+  const uint32_t t0 = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t pair0 = (int64_t)(g.rev ? gridDim.x - 1 - t0 : t0) * G;
   for (int f = threadIdx.x; f < G * N; f += blockDim.x) {
     const int gi = f / N, n = f - gi * N;
     const int64_t ra = 2 * (pair0 + gi), rb = ra + 1;
@@ -121,7 +122,8 @@ __global__ __launch_bounds__(kGenThreads) void k_gen_rows_c2r(T* data, RowGeom g
   const int N = sc.N, K = g.K;
   const int ls = N + 2;  // staging stride per pair (>= 2K)
   cx<T>* buf1 = buf0 + G * ls;
-  const int64_t pair0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * G;
+  const uint32_t t0 = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t pair0 = (int64_t)(g.rev ? gridDim.x - 1 - t0 : t0) * G;
   // stage both half spectra of every pair in buf1
   for (int f = threadIdx.x; f < G * K; f += blockDim.x) {
     const int gi = f / K, k = f - gi * K;
@@ -171,7 +173,8 @@ __global__ __launch_bounds__(kGenThreads) void k_gen_cols(const cx<T>* in, cx<T>
   cx<T>* buf0 = reinterpret_cast<cx<T>*>(raw);
   const int N = sc.N;
   cx<T>* buf1 = buf0 + N * CW;
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t t0 = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t tile = g.rev ? gridDim.x - 1 - t0 : t0;
   for (int f = threadIdx.x; f < N * CW; f += blockDim.x) {
     const int c = f % CW, n = f / CW;
     const uint32_t col = tile * CW + c;

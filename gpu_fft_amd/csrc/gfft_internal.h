@@ -36,6 +36,7 @@ struct ColGeom {
   uint32_t ncols;
   ColMap in, out;
   double scale;  // applied to the outputs (1 = unnormalised)
+  int32_t rev = 0;  // 1: tiles run in reverse order (last tile first)
 };
 
 inline ColMap col_map(uint32_t inner, int64_t outer, int64_t rlo) {
@@ -54,6 +55,7 @@ struct RowGeom {
   int64_t nrows;
   int32_t K;
   double scale;  // applied to the outputs (1 = unnormalised)
+  int32_t rev = 0;  // 1: tiles run in reverse order (last tile first)
 };
 
 constexpr int kMaxRadices = 16;

@@ -284,7 +284,8 @@ template <typename T, typename SC, int P, int G>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsR2C<T, SC, P, G>::NT))) void k_rows_r2c(T* data, RowGeom g, const cx<T>* tw) {
   using K_ = RowsR2C<T, SC, P, G>;
   __shared__ cx<T> smem[G * K_::LROW];
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t t0 = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t tile = g.rev ? gridDim.x - 1 - t0 : t0;
   K_::tile(data, g, tw, (int64_t)tile * G, smem);
 }
 
@@ -292,7 +293,8 @@ template <typename T, typename SC, int P, int G>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsC2R<T, SC, P, G>::NT))) void k_rows_c2r(T* data, RowGeom g, const cx<T>* tw) {
   using K_ = RowsC2R<T, SC, P, G>;
   __shared__ cx<T> smem[G * K_::LROW];
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t t0 = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t tile = g.rev ? gridDim.x - 1 - t0 : t0;
   K_::tile(data, g, tw, (int64_t)tile * G, smem);
 }
 
@@ -301,7 +303,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPass<T, INV, SC, P,
     const cx<T>* in, cx<T>* out, ColGeom g, const cx<T>* tw) {
   using K_ = ColsPass<T, INV, SC, P, CW>;
   __shared__ cx<T> smem[SC::N * CW];
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t t0 = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t tile = g.rev ? gridDim.x - 1 - t0 : t0;
   K_::tile(in, out, g, tw, tile, smem);
 }
 
@@ -316,15 +319,23 @@ using ColsPipe = ColsPass<T, INV, SC, P, CW, true>;
 // Tiles of a persistent workgroup: ORD 0 = the contiguous range
 // [w*T/G, (w+1)*T/G); ORD 1 = round robin w, w+G, w+2G, ... (neighbouring
 // tiles run concurrently on neighbouring workgroups of one XCD).
+// With rev, tile t of the walk is tile ntiles-1-t: the pass starts with the
+// tiles the previous pass wrote last (still in the memory-side cache).
 struct TileWalk {
-  uint32_t first, count, step;
-  __device__ __forceinline__ uint32_t at(uint32_t k) const { return first + k * step; }
+  uint32_t first, count, step, last;
+  int32_t rev;
+  __device__ __forceinline__ uint32_t at(uint32_t k) const {
+    const uint32_t t = first + k * step;
+    return rev ? last - t : t;
+  }
 };
 template <int ORD>
-__device__ __forceinline__ TileWalk tile_walk(uint32_t ntiles) {
+__device__ __forceinline__ TileWalk tile_walk(uint32_t ntiles, int32_t rev) {
   const uint32_t G = gridDim.x;
   const uint32_t w = xcd_tile(blockIdx.x, G);
   TileWalk tw;
+  tw.last = ntiles - 1;
+  tw.rev = rev;
   if (ORD == 0) {
     tw.first = (uint32_t)(((uint64_t)w * ntiles) / G);
     tw.count = (uint32_t)(((uint64_t)(w + 1) * ntiles) / G) - tw.first;
@@ -354,7 +365,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPipe<T, INV, SC, P,
   using K_ = ColsPipe<T, INV, SC, P, CW>;
   __shared__ cx<T> smem[SC::N * CW];
   __shared__ cx<T> stw[twiddle_count<SC>()];
-  const TileWalk walk = tile_walk<ORD>(ntiles);
+  const TileWalk walk = tile_walk<ORD>(ntiles, g.rev);
   if (walk.count == 0) return;  // uniform across the workgroup
   copy_twiddles<SC>(stw, tw, K_::NT);
   const int c = threadIdx.x % CW, t = threadIdx.x / CW;
@@ -379,7 +390,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsR2C<T, SC, P, G, tr
   using K_ = RowsR2C<T, SC, P, G, true>;
   __shared__ cx<T> smem[G * K_::LROW];
   __shared__ cx<T> stw[twiddle_count<SC>()];
-  const TileWalk walk = tile_walk<ORD>(ntiles);
+  const TileWalk walk = tile_walk<ORD>(ntiles, g.rev);
   if (walk.count == 0) return;
   copy_twiddles<SC>(stw, tw, K_::NT);
   const int gi = threadIdx.x / K_::TPT, t = threadIdx.x % K_::TPT;
@@ -401,7 +412,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsC2R<T, SC, P, G, tr
   using K_ = RowsC2R<T, SC, P, G, true>;
   __shared__ cx<T> smem[G * K_::LROW];
   __shared__ cx<T> stw[twiddle_count<SC>()];
-  const TileWalk walk = tile_walk<ORD>(ntiles);
+  const TileWalk walk = tile_walk<ORD>(ntiles, g.rev);
   if (walk.count == 0) return;
   copy_twiddles<SC>(stw, tw, K_::NT);
   const int gi = threadIdx.x / K_::TPT, t = threadIdx.x % K_::TPT;

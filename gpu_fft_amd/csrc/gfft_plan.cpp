@@ -400,13 +400,37 @@ double axis_scale(const PlanCore& c, int axis, bool inverse) {
   return axis == last ? c.r2c_scale : 1.0;
 }
 
+// Walk direction of a pass.  With alternating walks the passes of a round
+// trip run R2C z forward, y reversed, x forward; C2R x reversed, y forward,
+// z reversed: each pass starts on the tiles the previous pass wrote last,
+// which are still in the memory-side cache.
+bool reversed_walk(const PlanCore& c, int axis, bool inverse) {
+  return c.walk == 1 && (inverse ? axis != 1 : axis == 1);
+}
+
 int exec_axis(const PlanCore& c, int axis, bool inverse, void* data, hipStream_t s) {
   hipError_t e = hipSuccess;
   const double sc = axis_scale(c, axis, inverse);
+  const int32_t rev = reversed_walk(c, axis, inverse) ? 1 : 0;
   switch (axis) {
-    case 0: e = run_rows(c, c.z, inverse, data, z_geom(c, sc), s); break;
-    case 1: e = run_cols(c, c.y, inverse, data, y_geom(c, sc), s); break;
-    case 2: e = run_cols(c, c.x, inverse, data, x_geom(c, sc), s); break;
+    case 0: {
+      RowGeom g = z_geom(c, sc);
+      g.rev = rev;
+      e = run_rows(c, c.z, inverse, data, g, s);
+      break;
+    }
+    case 1: {
+      ColGeom g = y_geom(c, sc);
+      g.rev = rev;
+      e = run_cols(c, c.y, inverse, data, g, s);
+      break;
+    }
+    case 2: {
+      ColGeom g = x_geom(c, sc);
+      g.rev = rev;
+      e = run_cols(c, c.x, inverse, data, g, s);
+      break;
+    }
     default: return GFFT_ERR_INVALID_ARG;
   }
   return e == hipSuccess ? GFFT_OK : GFFT_ERR_HIP;

@@ -31,6 +31,7 @@ struct PlanCore {
   int procs = 1, rank = 0, prec = 0;
   double r2c_scale = 1.0, c2r_scale = 1.0;  // fused output scales (1 = unnormalised)
   int64_t chunk_planes = 0;  // > 0: z and y passes alternate per block of x-planes
+  int walk = 1;              // 1: passes alternate their tile order (see exec_axis)
   AxisPlan z, y, x;
 };
 

@@ -142,6 +142,13 @@ int gfft_plan_set_variant(gfft_plan plan, int axis, int variant);
  * default) runs whole-array passes.  Same results either way. */
 int gfft_plan_set_chunking(gfft_plan plan, int64_t planes);
 
+/* Tile order of the passes: 0 = every pass walks its tiles first to last;
+ * 1 (the default) = alternate, so that each pass of a round trip starts on
+ * the tiles the previous pass wrote last (R2C: z forward, y reversed, x
+ * forward; C2R: x reversed, y forward, z reversed), which are still in the
+ * memory-side cache.  Bitwise the same results either way. */
+int gfft_plan_set_walk(gfft_plan plan, int mode);
+
 /* Output scale factors fused into the last pass of each direction (default
  * 1.0 and 1.0: unnormalised, the reference's semantics).  c2r_scale =
  * 1/(Nx*Ny*Nz) gives the normalised inverse that the reference harness
