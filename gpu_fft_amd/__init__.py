@@ -292,6 +292,12 @@ def _bind_batch(L):
                                    ctypes.POINTER(ctypes.c_uint32)]
     L.gfft_batch_destroy.restype = ctypes.c_int
     L.gfft_batch_destroy.argtypes = [vp]
+    L.gfft_batch_tune.restype = ctypes.c_int
+    L.gfft_batch_tune.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.gfft_batch_set_variant.restype = ctypes.c_int
+    L.gfft_batch_set_variant.argtypes = [vp, i64, ctypes.c_int]
+    L.gfft_batch_variant.restype = ctypes.c_int
+    L.gfft_batch_variant.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_int)]
     L._gfft_batch_bound = True
     return L
 
@@ -339,6 +345,23 @@ class Batch:
 
     def exec_graph(self, mode=ROUND_TRIP, stream=None) -> None:
         _check(lib().gfft_batch_exec_graph(self.handle, int(mode), _stream_handle(stream)), "gfft_batch_exec_graph")
+
+    def tune(self, mode=ROUND_TRIP, reps: int = 3, stream=None) -> dict:
+        """Time every variant of every size class on the current data
+        (destroyed) and keep the fastest; returns {edge: variant}."""
+        _check(lib().gfft_batch_tune(self.handle, int(mode), int(reps), _stream_handle(stream)), "gfft_batch_tune")
+        return self.variants()
+
+    def set_variant(self, edge: int, variant: int) -> None:
+        _check(lib().gfft_batch_set_variant(self.handle, int(edge), int(variant)), "gfft_batch_set_variant")
+
+    def variants(self) -> dict:
+        out = {}
+        for n in sorted({d[2] for d in self.dims}):
+            v = ctypes.c_int()
+            _check(lib().gfft_batch_variant(self.handle, int(n), ctypes.byref(v)), "gfft_batch_variant")
+            out[n] = v.value
+        return out
 
     def tiles(self, mode=ROUND_TRIP):
         """(tiles, kernel launches) of one execution of ``mode``."""

@@ -12,8 +12,10 @@
 // branch per class).  There are no waits between workgroups, so a launch
 // always drains.
 //
-// Each tile runs the same register-engine tile code as the single-transform
-// kernels, configured so that every size class uses exactly 256 threads.
+// Each class has kVariants compiled variants (radix order, and points per
+// thread, which sets the pencils per 256-thread tile); gfft_batch_tune times
+// them on the batch's own data and keeps the fastest per class.  Every tile
+// runs the same register-engine tile code as the single-transform kernels.
 #include <algorithm>
 #include <new>
 #include <vector>
@@ -27,59 +29,55 @@ namespace {
 
 constexpr int kBT = 256;  // threads per workgroup, every tile kind
 constexpr int kClasses = 8;
+constexpr int kVariants = 3;
 constexpr int kMaxLevels = 6;
 constexpr int kLanes = 4;  // internal streams of the eager path
 
-// Size classes of the batched kernels: schedule and points per thread; rows
-// tiles carry 256 / (N/P) pairs, column tiles 256 / (N/P) columns.
-template <int C>
-struct BCls;
-template <>
-struct BCls<0> {
-  using SC = Sched<8, 4>;
-  static constexpr int P = 8;
-};
-template <>
-struct BCls<1> {
-  using SC = Sched<4, 4, 3>;
-  static constexpr int P = 12;
-};
-template <>
-struct BCls<2> {
-  using SC = Sched<8, 8>;
-  static constexpr int P = 8;
-};
-template <>
-struct BCls<3> {
-  using SC = Sched<8, 4, 3>;
-  static constexpr int P = 24;
-};
-template <>
-struct BCls<4> {
-  using SC = Sched<8, 4, 4>;
-  static constexpr int P = 8;
-};
-template <>
-struct BCls<5> {
-  using SC = Sched<4, 4, 2, 5>;
-  static constexpr int P = 20;
-};
-template <>
-struct BCls<6> {
-  using SC = Sched<8, 8, 3>;
-  static constexpr int P = 24;
-};
-template <>
-struct BCls<7> {
-  using SC = Sched<8, 8, 4>;
-  static constexpr int P = 8;
-};
+// Variant V of size class C: radix order and points per thread P; with
+// TPT = N/P threads per transform a tile carries W = 256/TPT row pairs (z)
+// or columns (y, x).
+template <int C, int V>
+struct BVar;
+#define GFFT_BVAR(C, V, P_, ...)     \
+  template <>                        \
+  struct BVar<C, V> {                \
+    using SC = Sched<__VA_ARGS__>;   \
+    static constexpr int P = P_;     \
+  };
+GFFT_BVAR(0, 0, 8, 8, 4)
+GFFT_BVAR(0, 1, 8, 4, 8)
+GFFT_BVAR(0, 2, 4, 8, 4)
+GFFT_BVAR(1, 0, 12, 4, 4, 3)
+GFFT_BVAR(1, 1, 12, 3, 4, 4)
+GFFT_BVAR(1, 2, 6, 4, 4, 3)
+GFFT_BVAR(2, 0, 8, 8, 8)
+GFFT_BVAR(2, 1, 4, 4, 4, 4)
+GFFT_BVAR(2, 2, 16, 8, 8)
+GFFT_BVAR(3, 0, 24, 8, 4, 3)
+GFFT_BVAR(3, 1, 12, 8, 4, 3)
+GFFT_BVAR(3, 2, 24, 3, 8, 4)
+GFFT_BVAR(4, 0, 8, 8, 4, 4)
+GFFT_BVAR(4, 1, 8, 4, 4, 8)
+GFFT_BVAR(4, 2, 16, 8, 4, 4)
+GFFT_BVAR(5, 0, 20, 4, 4, 2, 5)
+GFFT_BVAR(5, 1, 10, 8, 4, 5)
+GFFT_BVAR(5, 2, 20, 8, 4, 5)
+GFFT_BVAR(6, 0, 24, 8, 8, 3)
+GFFT_BVAR(6, 1, 12, 8, 8, 3)
+GFFT_BVAR(6, 2, 24, 3, 8, 8)
+GFFT_BVAR(7, 0, 8, 8, 8, 4)
+GFFT_BVAR(7, 1, 16, 8, 8, 4)
+GFFT_BVAR(7, 2, 8, 4, 8, 8)
+#undef GFFT_BVAR
 
-template <int C>
+constexpr int kClassN[kClasses] = {32, 48, 64, 96, 128, 160, 192, 256};
+
+template <int C, int V>
 struct BCfg {
-  using SC = typename BCls<C>::SC;
+  using SC = typename BVar<C, V>::SC;
   static constexpr int N = SC::N;
-  static constexpr int P = BCls<C>::P;
+  static_assert(N == kClassN[C], "variant size");
+  static constexpr int P = BVar<C, V>::P;
   static constexpr int TPT = N / P;
   static constexpr int W = kBT / TPT;  // pairs per rows tile = columns per cols tile
   static_assert(W * TPT == kBT, "tile must use all 256 threads");
@@ -94,31 +92,10 @@ struct BCfg {
   }
 };
 
-constexpr int kClassN[kClasses] = {32, 48, 64, 96, 128, 160, 192, 256};
-
 int class_of(int64_t n) {
   for (int c = 0; c < kClasses; ++c)
     if (kClassN[c] == n) return c;
   return -1;
-}
-
-Schedule class_sched(int c) {
-  switch (c) {
-    case 0: return make_sched<BCls<0>::SC>();
-    case 1: return make_sched<BCls<1>::SC>();
-    case 2: return make_sched<BCls<2>::SC>();
-    case 3: return make_sched<BCls<3>::SC>();
-    case 4: return make_sched<BCls<4>::SC>();
-    case 5: return make_sched<BCls<5>::SC>();
-    case 6: return make_sched<BCls<6>::SC>();
-    default: return make_sched<BCls<7>::SC>();
-  }
-}
-
-int class_width(int c) {
-  const int n = kClassN[c];
-  const int P[kClasses] = {8, 12, 8, 24, 8, 20, 24, 8};
-  return kBT / (n / P[c]);
 }
 
 // One transform of the batch.
@@ -137,7 +114,7 @@ struct BXform {
 struct LevelArgs {
   const BXform* xf;        // the class's transforms
   const uint32_t* prefix;  // [count + 1] tile prefix sums of this level
-  const cx<float>* tw;     // the class's twiddle table
+  const cx<float>* tw;     // twiddle table of the class's variant
   int32_t count;
   int32_t axis;            // 0 = z rows, 1 = y columns, 2 = x columns
   int32_t inv;
@@ -154,10 +131,9 @@ __device__ __forceinline__ ColMap dmap(uint32_t inner, int64_t outer, int64_t rl
   return m;
 }
 
-template <int C>
+template <typename F>
 __device__ __forceinline__ void run_tile(const BXform& x, int axis, bool inv, uint32_t tile,
                                          const cx<float>* tw, cx<float>* smem) {
-  using F = BCfg<C>;
   if (axis == 0) {
     RowGeom g;
     g.nrows = x.nrows;
@@ -189,9 +165,10 @@ __device__ __forceinline__ void run_tile(const BXform& x, int axis, bool inv, ui
 
 // Workgroups stride over the level's tiles; tile k belongs to the transform
 // t with prefix[t] <= k < prefix[t + 1].
-template <int C>
+template <int C, int V>
 __global__ __launch_bounds__(kBT) void k_batch_level(LevelArgs a) {
-  __shared__ cx<float> smem[BCfg<C>::lds_entries()];
+  using F = BCfg<C, V>;
+  __shared__ cx<float> smem[F::lds_entries()];
   for (uint32_t k = blockIdx.x; k < a.total; k += gridDim.x) {
     int lo = 0, hi = a.count;
     while (hi - lo > 1) {
@@ -201,36 +178,56 @@ __global__ __launch_bounds__(kBT) void k_batch_level(LevelArgs a) {
       else
         hi = mid;
     }
-    run_tile<C>(a.xf[lo], a.axis, a.inv != 0, k - a.prefix[lo], a.tw, smem);
+    run_tile<F>(a.xf[lo], a.axis, a.inv != 0, k - a.prefix[lo], a.tw, smem);
     __syncthreads();  // the tile's LDS reads are done before the next tile's writes
   }
 }
 
-const void* level_kernel(int c) {
-  switch (c) {
-    case 0: return reinterpret_cast<const void*>(&k_batch_level<0>);
-    case 1: return reinterpret_cast<const void*>(&k_batch_level<1>);
-    case 2: return reinterpret_cast<const void*>(&k_batch_level<2>);
-    case 3: return reinterpret_cast<const void*>(&k_batch_level<3>);
-    case 4: return reinterpret_cast<const void*>(&k_batch_level<4>);
-    case 5: return reinterpret_cast<const void*>(&k_batch_level<5>);
-    case 6: return reinterpret_cast<const void*>(&k_batch_level<6>);
-    default: return reinterpret_cast<const void*>(&k_batch_level<7>);
-  }
+using LaunchFn = hipError_t (*)(dim3, const LevelArgs&, hipStream_t);
+
+template <int C, int V>
+hipError_t launch_cv(dim3 grid, const LevelArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_batch_level<C, V>), grid, dim3(kBT), 0, s, a);
+  return hipGetLastError();
 }
 
-hipError_t launch_level(int c, dim3 grid, const LevelArgs& a, hipStream_t s) {
-  switch (c) {
-    case 0: hipLaunchKernelGGL(k_batch_level<0>, grid, dim3(kBT), 0, s, a); break;
-    case 1: hipLaunchKernelGGL(k_batch_level<1>, grid, dim3(kBT), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_batch_level<2>, grid, dim3(kBT), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(k_batch_level<3>, grid, dim3(kBT), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(k_batch_level<4>, grid, dim3(kBT), 0, s, a); break;
-    case 5: hipLaunchKernelGGL(k_batch_level<5>, grid, dim3(kBT), 0, s, a); break;
-    case 6: hipLaunchKernelGGL(k_batch_level<6>, grid, dim3(kBT), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_batch_level<7>, grid, dim3(kBT), 0, s, a); break;
+struct VarInfo {
+  Schedule sched;
+  int P, W;
+  const void* kernel;
+  LaunchFn launch;
+};
+
+template <int C, int V>
+VarInfo var_info() {
+  using F = BCfg<C, V>;
+  return VarInfo{make_sched<typename F::SC>(), F::P, F::W, reinterpret_cast<const void*>(&k_batch_level<C, V>),
+                 &launch_cv<C, V>};
+}
+
+template <int C>
+struct VarRow {
+  static void fill(VarInfo* row) {
+    row[0] = var_info<C, 0>();
+    row[1] = var_info<C, 1>();
+    row[2] = var_info<C, 2>();
   }
-  return hipGetLastError();
+};
+
+const VarInfo& var(int c, int v) {
+  static const auto tab = [] {
+    std::vector<VarInfo> t(kClasses * kVariants);
+    VarRow<0>::fill(&t[0 * kVariants]);
+    VarRow<1>::fill(&t[1 * kVariants]);
+    VarRow<2>::fill(&t[2 * kVariants]);
+    VarRow<3>::fill(&t[3 * kVariants]);
+    VarRow<4>::fill(&t[4 * kVariants]);
+    VarRow<5>::fill(&t[5 * kVariants]);
+    VarRow<6>::fill(&t[6 * kVariants]);
+    VarRow<7>::fill(&t[7 * kVariants]);
+    return t;
+  }();
+  return tab[c * kVariants + v];
 }
 
 }  // namespace
@@ -248,12 +245,13 @@ struct gfft_batch_s {
   std::vector<int> pos;        // caller index -> sorted position
   std::vector<BXform> hx;      // sorted by class
   int off[gfft::kClasses] = {}, n[gfft::kClasses] = {};
+  int var[gfft::kClasses] = {};  // chosen variant per class
   BXform* d_xf = nullptr;
-  uint32_t* d_prefix = nullptr;  // [class][mode][level][n_c + 1], packed
-  size_t prefix_off[gfft::kClasses][3] = {};
-  uint32_t level_tiles[gfft::kClasses][3][gfft::kMaxLevels] = {};
-  void* d_tw[gfft::kClasses] = {};
-  int occ[gfft::kClasses] = {};   // resident workgroups per CU of the class's kernel
+  uint32_t* d_prefix = nullptr;  // [class][variant][mode][level][n_c + 1], packed
+  size_t prefix_off[gfft::kClasses][gfft::kVariants][3] = {};
+  uint32_t level_tiles[gfft::kClasses][gfft::kVariants][3][gfft::kMaxLevels] = {};
+  void* d_tw[gfft::kClasses][gfft::kVariants] = {};
+  int occ[gfft::kClasses][gfft::kVariants] = {};  // resident workgroups per CU
   int lane[gfft::kClasses] = {};  // internal stream of the class's chain (eager path)
   int cus = 0;
   bool data_set = false;
@@ -270,20 +268,24 @@ const int kModeAxis[3][gfft::kMaxLevels] = {{0, 1, 2}, {2, 1, 0}, {0, 1, 2, 2, 1
 const int kModeInv[3][gfft::kMaxLevels] = {{0, 0, 0}, {1, 1, 1}, {0, 0, 0, 1, 1, 1}};
 const int kModeLevels[3] = {3, 3, 6};
 
-uint32_t tiles_of(const BXform& x, int axis) {
-  const int c = x.cls[axis];
-  if (c < 0) return 0;
-  const int w = gfft::class_width(c);
+uint32_t tiles_of(const BXform& x, int axis, int w) {
+  if (x.cls[axis] < 0) return 0;
   if (axis == 0) return (uint32_t)(((x.nrows + 1) / 2 + w - 1) / w);
   const uint32_t nc = axis == 1 ? x.ncols_y : x.ncols_x;
   return (nc + w - 1) / w;
 }
 
-void release(gfft_batch_s* b) {
+void drop_graphs(gfft_batch_s* b) {
   for (int m = 0; m < 3; ++m) {
     if (b->gexec[m]) (void)hipGraphExecDestroy(b->gexec[m]);
     if (b->graph[m]) (void)hipGraphDestroy(b->graph[m]);
+    b->gexec[m] = nullptr;
+    b->graph[m] = nullptr;
   }
+}
+
+void release(gfft_batch_s* b) {
+  drop_graphs(b);
   for (int i = 0; i < gfft::kLanes; ++i) {
     if (b->ev_join[i]) (void)hipEventDestroy(b->ev_join[i]);
     if (b->ws[i]) (void)hipStreamDestroy(b->ws[i]);
@@ -292,23 +294,35 @@ void release(gfft_batch_s* b) {
   if (b->d_xf) (void)hipFree(b->d_xf);
   if (b->d_prefix) (void)hipFree(b->d_prefix);
   for (int c = 0; c < gfft::kClasses; ++c)
-    if (b->d_tw[c]) (void)hipFree(b->d_tw[c]);
+    for (int v = 0; v < gfft::kVariants; ++v)
+      if (b->d_tw[c][v]) (void)hipFree(b->d_tw[c][v]);
 }
 
 gfft::LevelArgs level_args(const gfft_batch_s* b, int c, int mode, int l) {
+  const int v = b->var[c];
   gfft::LevelArgs a{};
   a.xf = b->d_xf + b->off[c];
-  a.prefix = b->d_prefix + b->prefix_off[c][mode] + (size_t)l * (b->n[c] + 1);
-  a.tw = static_cast<const gfft::cx<float>*>(b->d_tw[c]);
+  a.prefix = b->d_prefix + b->prefix_off[c][v][mode] + (size_t)l * (b->n[c] + 1);
+  a.tw = static_cast<const gfft::cx<float>*>(b->d_tw[c][v]);
   a.count = b->n[c];
   a.axis = kModeAxis[mode][l];
   a.inv = kModeInv[mode][l];
-  a.total = b->level_tiles[c][mode][l];
+  a.total = b->level_tiles[c][v][mode][l];
   return a;
 }
 
 uint32_t level_grid(const gfft_batch_s* b, int c, uint32_t total) {
-  return std::min<uint32_t>(total, (uint32_t)(b->cus * b->occ[c]));
+  return std::min<uint32_t>(total, (uint32_t)(b->cus * b->occ[c][b->var[c]]));
+}
+
+// The level chain of one class on one stream.
+int launch_chain(const gfft_batch_s* b, int c, int mode, hipStream_t s) {
+  for (int l = 0; l < kModeLevels[mode]; ++l) {
+    const gfft::LevelArgs a = level_args(b, c, mode, l);
+    if (a.total == 0) continue;
+    if (gfft::var(c, b->var[c]).launch(dim3(level_grid(b, c, a.total)), a, s) != hipSuccess) return GFFT_ERR_HIP;
+  }
+  return GFFT_OK;
 }
 
 // Eager path: fork the class chains onto the internal lanes, join back.
@@ -322,12 +336,8 @@ int launch_eager(gfft_batch_s* b, int mode, hipStream_t s) {
     if (used[i] && hipStreamWaitEvent(b->ws[i], b->ev_fork, 0) != hipSuccess) return GFFT_ERR_HIP;
   for (int c = 0; c < kClasses; ++c) {
     if (b->n[c] == 0) continue;
-    for (int l = 0; l < kModeLevels[mode]; ++l) {
-      const LevelArgs a = level_args(b, c, mode, l);
-      if (a.total == 0) continue;
-      if (launch_level(c, dim3(level_grid(b, c, a.total)), a, b->ws[b->lane[c]]) != hipSuccess)
-        return GFFT_ERR_HIP;
-    }
+    const int st = launch_chain(b, c, mode, b->ws[b->lane[c]]);
+    if (st != GFFT_OK) return st;
   }
   for (int i = 0; i < kLanes; ++i) {
     if (!used[i]) continue;
@@ -350,7 +360,7 @@ int build_graph(gfft_batch_s* b, int mode, hipGraph_t* out) {
       if (a.total == 0) continue;
       void* params[] = {&a};
       hipKernelNodeParams p{};
-      p.func = const_cast<void*>(level_kernel(c));
+      p.func = const_cast<void*>(var(c, b->var[c]).kernel);
       p.gridDim = dim3(level_grid(b, c, a.total));
       p.blockDim = dim3(kBT);
       p.sharedMemBytes = 0;
@@ -442,19 +452,21 @@ int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec)
     work[x.cls[0]] += (double)b->bytes[i];
   }
   for (int c = 1; c < kClasses; ++c) b->off[c] = b->off[c - 1] + b->n[c - 1];
-  // tile layout per class, mode and level
+  // tile layout per class, variant, mode and level
   std::vector<uint32_t> prefix;
   for (int c = 0; c < kClasses; ++c)
-    for (int m = 0; m < 3; ++m) {
-      b->prefix_off[c][m] = prefix.size();
-      for (int l = 0; l < kModeLevels[m]; ++l) {
-        const size_t base = prefix.size();
-        prefix.resize(base + b->n[c] + 1, 0);
-        for (int k = 0; k < b->n[c]; ++k)
-          prefix[base + k + 1] = prefix[base + k] + tiles_of(b->hx[b->off[c] + k], kModeAxis[m][l]);
-        b->level_tiles[c][m][l] = prefix[base + b->n[c]];
+    for (int v = 0; v < kVariants; ++v)
+      for (int m = 0; m < 3; ++m) {
+        b->prefix_off[c][v][m] = prefix.size();
+        const int w = var(c, v).W;
+        for (int l = 0; l < kModeLevels[m]; ++l) {
+          const size_t base = prefix.size();
+          prefix.resize(base + b->n[c] + 1, 0);
+          for (int k = 0; k < b->n[c]; ++k)
+            prefix[base + k + 1] = prefix[base + k] + tiles_of(b->hx[b->off[c] + k], kModeAxis[m][l], w);
+          b->level_tiles[c][v][m][l] = prefix[base + b->n[c]];
+        }
       }
-    }
   // class chains onto lanes: heaviest first, each to the least loaded lane
   {
     int cl[kClasses];
@@ -481,15 +493,17 @@ int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec)
   alloc((void**)&b->d_prefix, sizeof(uint32_t) * std::max<size_t>(prefix.size(), 1));
   for (int c = 0; c < kClasses && st == GFFT_OK; ++c) {
     if (b->n[c] == 0) continue;
-    const std::vector<unsigned char> tw = twiddle_table(class_sched(c), F32);
-    alloc(&b->d_tw[c], tw.size());
-    if (st == GFFT_OK && hipMemcpy(b->d_tw[c], tw.data(), tw.size(), hipMemcpyHostToDevice) != hipSuccess)
-      st = GFFT_ERR_HIP;
-    int occ = 0;
-    if (st == GFFT_OK &&
-        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, level_kernel(c), kBT, 0) != hipSuccess || occ < 1))
-      occ = 1;
-    b->occ[c] = occ;
+    for (int v = 0; v < kVariants && st == GFFT_OK; ++v) {
+      const std::vector<unsigned char> tw = twiddle_table(var(c, v).sched, F32);
+      alloc(&b->d_tw[c][v], tw.size());
+      if (st == GFFT_OK && hipMemcpy(b->d_tw[c][v], tw.data(), tw.size(), hipMemcpyHostToDevice) != hipSuccess)
+        st = GFFT_ERR_HIP;
+      int occ = 0;
+      if (st == GFFT_OK &&
+          (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, var(c, v).kernel, kBT, 0) != hipSuccess || occ < 1))
+        occ = 1;
+      b->occ[c][v] = occ;
+    }
   }
   for (int i = 0; i < kLanes && st == GFFT_OK; ++i) {
     if (hipStreamCreateWithFlags(&b->ws[i], hipStreamNonBlocking) != hipSuccess ||
@@ -556,6 +570,62 @@ int gfft_batch_exec_graph(gfft_batch b, int mode, gfft_stream_t stream) {
                                                                                             : GFFT_ERR_HIP;
 }
 
+int gfft_batch_set_variant(gfft_batch b, int64_t edge, int variant) {
+  if (!b || variant < 0 || variant >= gfft::kVariants) return GFFT_ERR_INVALID_ARG;
+  const int c = gfft::class_of(edge);
+  if (c < 0) return GFFT_ERR_INVALID_ARG;
+  if (hipDeviceSynchronize() != hipSuccess) return GFFT_ERR_HIP;
+  b->var[c] = variant;
+  drop_graphs(b);
+  return GFFT_OK;
+}
+
+int gfft_batch_variant(gfft_batch b, int64_t edge, int* variant) {
+  if (!b || !variant) return GFFT_ERR_INVALID_ARG;
+  const int c = gfft::class_of(edge);
+  if (c < 0) return GFFT_ERR_INVALID_ARG;
+  *variant = b->var[c];
+  return GFFT_OK;
+}
+
+int gfft_batch_tune(gfft_batch b, int mode, int reps, gfft_stream_t stream) {
+  if (!b || mode < 0 || mode > 2 || !b->data_set) return GFFT_ERR_INVALID_ARG;
+  int st = check_dev(b);
+  if (st != GFFT_OK) return st;
+  if (reps < 1) reps = 1;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess) return GFFT_ERR_HIP;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return GFFT_ERR_HIP;
+  }
+  for (int c = 0; c < gfft::kClasses && st == GFFT_OK; ++c) {
+    if (b->n[c] == 0) continue;
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v < gfft::kVariants && st == GFFT_OK; ++v) {
+      b->var[c] = v;
+      if ((st = launch_chain(b, c, mode, s)) != GFFT_OK) break;  // warm-up
+      if (hipEventRecord(e0, s) != hipSuccess) st = GFFT_ERR_HIP;
+      for (int r = 0; r < reps && st == GFFT_OK; ++r) st = launch_chain(b, c, mode, s);
+      float ms = 0;
+      if (st == GFFT_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = GFFT_ERR_HIP;
+      if (st == GFFT_OK && ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    b->var[c] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  drop_graphs(b);
+  return st;
+}
+
 int gfft_batch_data_bytes(gfft_batch b, int index, size_t* bytes) {
   if (!b || !bytes || index < 0 || index >= b->count) return GFFT_ERR_INVALID_ARG;
   *bytes = b->bytes[index];
@@ -567,8 +637,9 @@ int gfft_batch_tiles(gfft_batch b, int mode, uint32_t* tiles, uint32_t* launches
   uint32_t t = 0, n = 0;
   for (int c = 0; c < gfft::kClasses; ++c)
     for (int l = 0; l < kModeLevels[mode]; ++l) {
-      t += b->level_tiles[c][mode][l];
-      n += b->level_tiles[c][mode][l] ? 1u : 0u;
+      const uint32_t k = b->level_tiles[c][b->var[c]][mode][l];
+      t += k;
+      n += k ? 1u : 0u;
     }
   *tiles = t;
   if (launches) *launches = n;

@@ -87,15 +87,14 @@ bool same_sched(const Schedule& a, const Schedule& b) {
   return true;
 }
 
-// All compiled (forward, inverse) pairs for a size, default first; only
-// variants with the default's radix order (one twiddle table per axis).
+// All compiled (forward, inverse) pairs for a size, default first (any radix
+// order: the plan keeps one twiddle table per distinct order).
 std::vector<std::pair<const KernelEntry*, const KernelEntry*>> find_pairs(int prec, int kf, int ki, int n) {
   std::vector<std::pair<const KernelEntry*, const KernelEntry*>> out;
   const KernelEntry* tab = nullptr;
   const int cnt = registry(prec, &tab);
   for (int i = 0; i < cnt; ++i) {
     if (tab[i].kind != kf || tab[i].N != n) continue;
-    if (!out.empty() && !same_sched(out[0].first->sched, tab[i].sched)) continue;
     for (int j = 0; j < cnt; ++j)
       if (tab[j].kind == ki && tab[j].N == n && tab[j].P == tab[i].P && tab[j].tile == tab[i].tile &&
           tab[j].persist == tab[i].persist && tab[j].order == tab[i].order &&
@@ -161,21 +160,33 @@ int plan_axis(AxisPlan& a, int prec, int64_t n, bool rows, int64_t K) {
   const int kf = rows ? ROWS_R2C : COLS_FWD;
   const int ki = rows ? ROWS_C2R : COLS_INV;
   a.cands = find_pairs(prec, kf, ki, (int)n);
+  Schedule def{};
   if (!a.cands.empty()) {
-    a.sched = a.cands[0].first->sched;
+    def = a.cands[0].first->sched;
   } else {
     if ((int)rad.size() > kMaxRadices) return GFFT_ERR_UNSUPPORTED_SIZE;
-    a.sched.N = (int)n;
-    a.sched.S = (int)rad.size();
-    for (size_t i = 0; i < rad.size(); ++i) a.sched.R[i] = rad[i];
+    def.N = (int)n;
+    def.S = (int)rad.size();
+    for (size_t i = 0; i < rad.size(); ++i) def.R[i] = rad[i];
+  }
+  a.scheds.push_back(def);
+  for (const auto& cd : a.cands) {
+    int k = 0;
+    while (k < (int)a.scheds.size() && !same_sched(a.scheds[k], cd.first->sched)) ++k;
+    if (k == (int)a.scheds.size()) a.scheds.push_back(cd.first->sched);
+    a.cand_sched.push_back(k);
   }
   const size_t lds = rows ? generic_rows_lds(prec, (int)n, (int)K) : generic_cols_lds(prec, (int)n);
-  if (lds != 0) a.cands.emplace_back(nullptr, nullptr);
+  if (lds != 0) {
+    a.cands.emplace_back(nullptr, nullptr);  // the generic kernel runs the default order
+    a.cand_sched.push_back(0);
+  }
   if (a.cands.empty()) return GFFT_ERR_UNSUPPORTED_SIZE;
   a.variant = 0;
   a.reg_fwd = a.cands[0].first;
   a.reg_inv = a.cands[0].second;
-  a.rt = to_rt(a.sched);
+  a.sched = def;
+  a.rt = to_rt(def);
   return GFFT_OK;
 }
 
@@ -250,8 +261,11 @@ int plan_host(PlanCore& c, int64_t nx, int64_t ny, int64_t nz, int procs, int ra
   const size_t cs = prec == F32 ? 8 : 16;
   for (AxisPlan* a : {&c.z, &c.y, &c.x}) {
     if (!a->active) continue;
-    const size_t n = prec == F32 ? make_twiddles<float>(a->sched).size() : make_twiddles<double>(a->sched).size();
-    a->tw_bytes = n / 2 * cs;
+    a->tw_bytes = 0;
+    for (const Schedule& sc : a->scheds) {
+      const size_t n = prec == F32 ? make_twiddles<float>(sc).size() : make_twiddles<double>(sc).size();
+      a->tw_bytes += n / 2 * cs;
+    }
   }
   return GFFT_OK;
 }
@@ -268,34 +282,28 @@ std::string plan_json(const PlanCore& c) {
 int upload_twiddles(PlanCore& c) {
   for (AxisPlan* a : {&c.z, &c.y, &c.x}) {
     if (!a->active) continue;
-    size_t bytes = 0;
-    const void* src = nullptr;
-    std::vector<float> tf;
-    std::vector<double> td;
-    if (c.prec == F32) {
-      tf = make_twiddles<float>(a->sched);
-      src = tf.data();
-      bytes = tf.size() * sizeof(float);
-    } else {
-      td = make_twiddles<double>(a->sched);
-      src = td.data();
-      bytes = td.size() * sizeof(double);
+    a->tw_bytes = 0;
+    for (const Schedule& sc : a->scheds) {
+      const std::vector<unsigned char> t = twiddle_table(sc, c.prec);
+      void* d = nullptr;
+      if (hipMalloc(&d, t.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        return GFFT_ERR_OUT_OF_MEMORY;
+      }
+      a->d_tws.push_back(d);
+      a->tw_bytes += t.size();
+      if (hipMemcpy(d, t.data(), t.size(), hipMemcpyHostToDevice) != hipSuccess) return GFFT_ERR_HIP;
     }
-    void* d = nullptr;
-    if (hipMalloc(&d, bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      return GFFT_ERR_OUT_OF_MEMORY;
-    }
-    a->d_tw = d;
-    a->tw_bytes = bytes;
-    if (hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return GFFT_ERR_HIP;
+    a->d_tw = a->d_tws[a->cand_sched[a->variant]];
   }
   return GFFT_OK;
 }
 
 void free_twiddles(PlanCore& c) {
   for (AxisPlan* a : {&c.z, &c.y, &c.x}) {
-    if (a->d_tw) (void)hipFree(a->d_tw);
+    for (void* d : a->d_tws)
+      if (d) (void)hipFree(d);
+    a->d_tws.clear();
     a->d_tw = nullptr;
   }
 }
@@ -446,6 +454,8 @@ int set_variant(PlanCore& c, int axis, int variant) {
   a->variant = variant;
   a->reg_fwd = a->cands[variant].first;
   a->reg_inv = a->cands[variant].second;
+  a->sched = a->scheds[a->cand_sched[variant]];
+  if (!a->d_tws.empty()) a->d_tw = a->d_tws[a->cand_sched[variant]];
   a->tuned_ms = -1;
   return GFFT_OK;
 }

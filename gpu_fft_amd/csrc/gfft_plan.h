@@ -20,10 +20,15 @@ struct AxisPlan {
   std::vector<std::pair<const KernelEntry*, const KernelEntry*>> cands;
   int variant = 0;       // index into cands
   double tuned_ms = -1;  // fwd+inv time of the chosen variant when tuned
-  Schedule sched{};      // radix order used (fixes the twiddle table)
-  RtSchedHost rt{};      // the same order for the generic kernels
-  void* d_tw = nullptr;  // device twiddle table
-  size_t tw_bytes = 0;
+  Schedule sched{};      // radix order of the current variant
+  RtSchedHost rt{};      // radix order of the generic kernel (scheds[0])
+  // distinct radix orders among the candidates (scheds[0] = the default's)
+  // and, per candidate, the index of its order; one twiddle table per order
+  std::vector<Schedule> scheds;
+  std::vector<int> cand_sched;
+  std::vector<void*> d_tws;
+  void* d_tw = nullptr;  // device twiddle table of the current variant
+  size_t tw_bytes = 0;   // all tables of this axis
 };
 
 struct PlanCore {

@@ -20,6 +20,12 @@ using Sc768 = Sched<8, 8, 4, 3>;
 using Sc1024 = Sched<8, 8, 4, 4>;
 using Sc2048 = Sched<8, 8, 8, 4>;
 
+// Alternative radix orders (tuning candidates; each has its own twiddle table).
+using Sc768b = Sched<3, 8, 8, 4>;
+using Sc768c = Sched<8, 4, 8, 3>;
+using Sc1024b = Sched<4, 4, 8, 8>;
+using Sc1024c = Sched<8, 4, 4, 8>;
+
 // 2^a 3^b 5^c sizes of the serving range [64, 512] without a power-of-two
 // schedule above: 8s first (a 2^4 factor as 4 x 4), then 5s, then 3s.
 using Sc72 = Sched<8, 3, 3>;

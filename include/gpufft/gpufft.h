@@ -209,6 +209,19 @@ int gfft_batch_exec(gfft_batch batch, int mode, gfft_stream_t stream);
  * effect in replays. */
 int gfft_batch_exec_graph(gfft_batch batch, int mode, gfft_stream_t stream);
 
+/* Autotuner of the batched kernels: every size class present has compiled
+ * variants (radix order; points per thread, which sets the pencils per
+ * 256-thread tile).  Times each variant's level chain of `mode`, `reps`
+ * times, on the arrays set by gfft_batch_set_data (contents destroyed) and
+ * keeps the fastest per class.  Synchronises `stream`; later graph launches
+ * are rebuilt with the chosen kernels. */
+int gfft_batch_tune(gfft_batch batch, int mode, int reps, gfft_stream_t stream);
+
+/* Select variant `variant` (0 = default) of the size class with edge `edge`;
+ * query the current choice.  set synchronises the device. */
+int gfft_batch_set_variant(gfft_batch batch, int64_t edge, int variant);
+int gfft_batch_variant(gfft_batch batch, int64_t edge, int* variant);
+
 int gfft_batch_data_bytes(gfft_batch batch, int index, size_t* bytes);
 
 /* Tiles one execution of `mode` processes, and (if launches is not NULL)

@@ -159,3 +159,42 @@ def test_batch_rejects_unsupported_sizes():
         G.Batch([(32, 32, 32), (40, 40, 40)])
     with pytest.raises(G.GfftError):
         G.Batch([(32, 64, 32)])
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_every_variant_matches_oracle(variant):
+    """Each compiled variant (radix order, pencils per tile) of every size
+    class, forced for all classes, against the FP64 oracle."""
+    import torch
+    dims = [(n, n, n) for n in SIZES if n <= 192] + [(1, 256, 256), (256, 1, 256)]
+    bufs, host = inputs(torch, dims, 300 + variant)
+    say(f"case batch variant {variant}, {len(dims)} transforms")
+    with G.Batch(dims) as b:
+        for n in SIZES:
+            b.set_variant(n, variant)
+        assert set(b.variants().values()) == {variant}
+        b.set_data(bufs)
+        b.exec(G.R2C)
+        torch.cuda.synchronize()
+        for (nx, ny, nz), buf, fp in zip(dims, bufs, host):
+            got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
+            err = O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), nz))
+            assert err <= 1e-6, ((nx, ny, nz), variant, err)
+
+
+def test_tuned_batch_matches_oracle():
+    import torch
+    dims = mix(24, seed=4, max_edge=128) + [(160, 160, 160), (1, 256, 256)]
+    scratch, _ = inputs(torch, dims, 500)
+    with G.Batch(dims) as b:
+        b.set_data(scratch)
+        chosen = b.tune(G.ROUND_TRIP, reps=2)
+        assert set(chosen) == {d[2] for d in dims} and all(0 <= v < 3 for v in chosen.values())
+        bufs, host = inputs(torch, dims, 600)
+        b.set_data(bufs)
+        b.exec_graph(G.R2C)
+        torch.cuda.synchronize()
+        for (nx, ny, nz), buf, fp in zip(dims, bufs, host):
+            got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
+            err = O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), nz))
+            assert err <= 1e-6, ((nx, ny, nz), err)
