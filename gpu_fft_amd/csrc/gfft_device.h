@@ -242,9 +242,26 @@ __device__ __forceinline__ void lds_barrier() {
   __asm__ volatile("" ::: "memory");
 }
 
-template <bool LB>
+// Exchange between the threads of one wavefront only: LDS operations of a
+// wave complete in program order, so waiting for this wave's own LDS traffic
+// (lgkmcnt(0)) and keeping the compiler from moving memory accesses across
+// the point is enough; other waves are not waited for.
+__device__ __forceinline__ void wave_sync() {
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_wave_barrier();
+  __asm__ volatile("" ::: "memory");
+}
+
+// Barrier between the stages of the register engine.  BM 0: workgroup
+// barrier; 1: workgroup barrier on LDS traffic only (prefetched global loads
+// stay in flight); 2: wave-local (every thread sharing the LDS region is in
+// the calling thread's wavefront).
+template <int BM>
 __device__ __forceinline__ void stage_barrier() {
-  if constexpr (LB)
+  if constexpr (BM == 2)
+    wave_sync();
+  else if constexpr (BM == 1)
     lds_barrier();
   else
     __syncthreads();
@@ -260,7 +277,7 @@ __device__ __forceinline__ void stage_barrier() {
 // After run(), v[q*RL + r] holds X[t + TPT*q + r*N/RL] (valid slots).
 // LB selects LDS-only barriers between stages (see lds_barrier).
 // ---------------------------------------------------------------------------
-template <typename T, bool INV, typename SC, int P, bool LB = false>
+template <typename T, bool INV, typename SC, int P, int LB = 0>
 struct RegEngine {
   using C = cx<T>;
   static constexpr int N = SC::N;

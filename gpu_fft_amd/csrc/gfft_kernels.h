@@ -43,9 +43,17 @@ struct ColLds {
 // ---------------------------------------------------------------------------
 // z-axis R2C, register engine
 // ---------------------------------------------------------------------------
+// Rows tiles: thread gi*TPT + t works on row pair gi, so when TPT divides 64
+// every LDS exchange of a pair stays inside one wavefront (barrier mode 2).
+template <int N, int P, bool LB>
+constexpr int rows_barrier() {
+  return (64 % (N / P) == 0) ? 2 : (LB ? 1 : 0);
+}
+
 template <typename T, typename SC, int P, int G, bool LB = false>
 struct RowsR2C {
-  using E = RegEngine<T, false, SC, P, LB>;
+  static constexpr int BM = rows_barrier<SC::N, P, LB>();
+  using E = RegEngine<T, false, SC, P, BM>;
   using C = cx<T>;
   static constexpr int N = SC::N;
   static constexpr int TPT = E::TPT;
@@ -76,14 +84,14 @@ struct RowsR2C {
   // stored over the pair's own rows
   __device__ __forceinline__ static void finish(C (&v)[E::PV], T* __restrict__ data, const RowGeom& g,
                                                 int64_t pair, int t, RowLds<T, N>& L) {
-    stage_barrier<LB>();
+    stage_barrier<BM>();
 #pragma unroll
     for (int q = 0; q < E::NBL; ++q) {
       if (!E::liveL(t, q)) continue;
 #pragma unroll
       for (int r = 0; r < E::RL; ++r) L.at(E::out_index(t, q, r)) = v[q * E::RL + r];
     }
-    stage_barrier<LB>();
+    stage_barrier<BM>();
     const int64_t ra = 2 * pair, rb = ra + 1;
     const bool va = ra < g.nrows, vb = rb < g.nrows;
     C* Ac = reinterpret_cast<C*>(data) + ra * KC;
@@ -119,7 +127,8 @@ struct RowsR2C {
 // ---------------------------------------------------------------------------
 template <typename T, typename SC, int P, int G, bool LB = false>
 struct RowsC2R {
-  using E = RegEngine<T, true, SC, P, LB>;
+  static constexpr int BM = rows_barrier<SC::N, P, LB>();
+  using E = RegEngine<T, true, SC, P, BM>;
   using C = cx<T>;
   static constexpr int N = SC::N;
   static constexpr int TPT = E::TPT;
@@ -167,7 +176,7 @@ struct RowsC2R {
         S[KC + k] = xb[i];
       }
     }
-    stage_barrier<LB>();
+    stage_barrier<BM>();
 #pragma unroll
     for (int q = 0; q < E::NB0; ++q) {
       if (!E::live0(t, q)) continue;
@@ -393,6 +402,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsR2C<T, SC, P, G, tr
   const TileWalk walk = tile_walk<ORD>(ntiles, g.rev);
   if (walk.count == 0) return;
   copy_twiddles<SC>(stw, tw, K_::NT);
+  __syncthreads();  // the table is complete before any wave's first stage
   const int gi = threadIdx.x / K_::TPT, t = threadIdx.x % K_::TPT;
   RowLds<T, SC::N> L{smem + gi * K_::LROW};
   cx<T> cur[K_::E::PV], nxt[K_::E::PV];
@@ -415,6 +425,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsC2R<T, SC, P, G, tr
   const TileWalk walk = tile_walk<ORD>(ntiles, g.rev);
   if (walk.count == 0) return;
   copy_twiddles<SC>(stw, tw, K_::NT);
+  __syncthreads();  // the table is complete before any wave's first stage
   const int gi = threadIdx.x / K_::TPT, t = threadIdx.x % K_::TPT;
   cx<T>* S = smem + gi * K_::LROW;
   RowLds<T, SC::N> L{S};
@@ -427,7 +438,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsC2R<T, SC, P, G, tr
       cb[i] = nb[i];
     }
     if (k + 1 < walk.count) K_::load(na, nb, data, g, (int64_t)walk.at(k + 1) * G + gi, t);
-    lds_barrier();  // the previous tile's stage reads of S are done
+    stage_barrier<K_::BM>();  // the previous tile's stage reads of S are done
     K_::build(v, ca, cb, S, t);
     K_::E::run(v, t, L, stw);
     K_::store(v, data, g, (int64_t)walk.at(k) * G + gi, t);
