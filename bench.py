@@ -387,7 +387,8 @@ def main():
                     help="default run without the batched and served sub-results")
     ap.add_argument("--prec", default="f32", choices=["f32", "f64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=40,
+                    help="CPU baseline repetitions (40 x ~0.2 s on 16 threads: ~8 s wall, ~2 CPU-minutes)")
     ap.add_argument("--no-tune", action="store_true", help="use the default kernel variants")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of S bytes (HBM counter calibration under rocprofv3)")

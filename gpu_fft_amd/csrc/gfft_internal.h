@@ -84,7 +84,8 @@ struct KernelEntry {
   int threads;      // threads per workgroup
   void* fn;         // RowsLaunchFn or ColsLaunchFn
   int persist;      // 0: one tile per workgroup; k: persistent, k workgroups per CU
-  int order;        // persistent tile order: 0 contiguous ranges, 1 round robin
+  int order;        // tile order: persistent 0 contiguous ranges, 1 round robin; one tile per
+                    // workgroup 0 XCD-aware ranges, 2 linear
 };
 
 // Registry units of compiled variants; each returns its number of entries.

@@ -307,12 +307,16 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsC2R<T, SC, P, G>::N
   K_::tile(data, g, tw, (int64_t)tile * G, smem);
 }
 
-template <typename T, bool INV, typename SC, int P, int CW>
+// ORD 0: XCD-aware tile order (each XCD walks a contiguous range, so tiles
+// that share straddled cache lines meet in one L2).  ORD 2: tile = workgroup
+// id, so the tiles in flight form one contiguous range across all XCDs (for
+// line-aligned tiles: the rows they read are then read densely).
+template <typename T, bool INV, typename SC, int P, int CW, int ORD = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPass<T, INV, SC, P, CW>::NT))) void k_cols(
     const cx<T>* in, cx<T>* out, ColGeom g, const cx<T>* tw) {
   using K_ = ColsPass<T, INV, SC, P, CW>;
   __shared__ cx<T> smem[SC::N * CW];
-  const uint32_t t0 = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t t0 = ORD == 2 ? blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t tile = g.rev ? gridDim.x - 1 - t0 : t0;
   K_::tile(in, out, g, tw, tile, smem);
 }
@@ -468,12 +472,12 @@ hipError_t launch_rows_c2r(void* data, RowGeom g, const void* tw, hipStream_t s)
   return hipGetLastError();
 }
 
-template <typename T, bool INV, typename SC, int P, int CW>
+template <typename T, bool INV, typename SC, int P, int CW, int ORD = 0>
 hipError_t launch_cols(const void* in, void* out, ColGeom g, const void* tw, hipStream_t s) {
   const int64_t blocks = ((int64_t)g.ncols + CW - 1) / CW;
   if (blocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_cols<T, INV, SC, P, CW>), dim3((uint32_t)blocks), dim3(ColsPass<T, INV, SC, P, CW>::NT), 0,
-                     s, static_cast<const cx<T>*>(in), static_cast<cx<T>*>(out), g, static_cast<const cx<T>*>(tw));
+  hipLaunchKernelGGL((k_cols<T, INV, SC, P, CW, ORD>), dim3((uint32_t)blocks), dim3(ColsPass<T, INV, SC, P, CW>::NT),
+                     0, s, static_cast<const cx<T>*>(in), static_cast<cx<T>*>(out), g, static_cast<const cx<T>*>(tw));
   return hipGetLastError();
 }
 
@@ -538,10 +542,11 @@ KernelEntry entry_rows_c2r() {
   return KernelEntry{prec_of<T>(), ROWS_C2R, SC::N, make_sched<SC>(), P, G, RowsC2R<T, SC, P, G>::NT,
                      reinterpret_cast<void*>(&launch_rows_c2r<T, SC, P, G>)};
 }
-template <typename T, bool INV, typename SC, int P, int CW>
+template <typename T, bool INV, typename SC, int P, int CW, int ORD = 0>
 KernelEntry entry_cols() {
   return KernelEntry{prec_of<T>(), INV ? COLS_INV : COLS_FWD, SC::N, make_sched<SC>(), P, CW,
-                     ColsPass<T, INV, SC, P, CW>::NT, reinterpret_cast<void*>(&launch_cols<T, INV, SC, P, CW>)};
+                     ColsPass<T, INV, SC, P, CW>::NT,
+                     reinterpret_cast<void*>(&launch_cols<T, INV, SC, P, CW, ORD>), 0, ORD};
 }
 
 template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD>

@@ -61,6 +61,15 @@ int registry_f32(const KernelEntry** out) {
       GFFT_ROWS_PIPE(float, Sc1024, 8, 2, 2, 1),
       GFFT_ROWS_PIPE(float, Sc2048, 8, 1, 4, 1),
       GFFT_ROWS_PIPE(float, Sc768, 24, 4, 2, 1),
+      // linear tile dispatch
+      GFFT_COLS_LIN(float, Sc512, 16, 16),
+      GFFT_COLS_LIN(float, Sc512, 8, 16),
+      GFFT_COLS_LIN(float, Sc768, 24, 16),
+      GFFT_COLS_LIN(float, Sc768, 24, 8),
+      GFFT_COLS_LIN(float, Sc1024, 16, 8),
+      GFFT_COLS_LIN(float, Sc1024, 16, 16),
+      GFFT_COLS_LIN(float, Sc1024, 32, 16),
+      GFFT_COLS_LIN(float, Sc256, 8, 16),
       // other radix orders
       GFFT_ROWS(float, Sc1024b, 8, 2),
       GFFT_COLS_PIPE(float, Sc1024b, 16, 16, 1, 0),

@@ -69,6 +69,9 @@ using Sc500 = Sched<4, 5, 5, 5>;
 #define GFFT_SIZE(T, SC, PR, G, PC, CW) GFFT_ROWS(T, SC, PR, G), GFFT_COLS(T, SC, PC, CW)
 #define GFFT_ROWS(T, SC, PR, G) entry_rows_r2c<T, SC, PR, G>(), entry_rows_c2r<T, SC, PR, G>()
 #define GFFT_COLS(T, SC, PC, CW) entry_cols<T, false, SC, PC, CW>(), entry_cols<T, true, SC, PC, CW>()
+// the same with linear tile dispatch (no XCD-aware remap)
+#define GFFT_COLS_LIN(T, SC, PC, CW) \
+  entry_cols<T, false, SC, PC, CW, 2>(), entry_cols<T, true, SC, PC, CW, 2>()
 // persistent software-pipelined column pass, OCC workgroups per CU
 #define GFFT_COLS_PIPE(T, SC, PC, CW, OCC, ORD) \
   entry_cols_pipe<T, false, SC, PC, CW, OCC, ORD>(), entry_cols_pipe<T, true, SC, PC, CW, OCC, ORD>()
