@@ -89,6 +89,7 @@ struct gfft_server_s {
   std::vector<Lane> lanes;  // one per stream
   size_t n_running = 0;
   std::map<Key, std::unique_ptr<gfft::PlanCore>> plans;
+  std::map<Key, int> checked;  // host-side plan status of geometries seen at submission
   size_t plan_bytes = 0;
   std::vector<Events> ev_pool;
   std::vector<int> ev_free;
@@ -374,6 +375,18 @@ int gfft_server_submit(gfft_server s, const gfft_request* req, uint64_t* id) {
   size_t bytes = 0;
   int st = request_bytes(req->nx, req->ny, req->nz, req->prec, &bytes);
   if (st != GFFT_OK) return st;
+  {
+    // sizes the planner cannot serve (a prime factor > 5, ...) fail here,
+    // not at admission; the host-only check is cached per geometry
+    const Key k{req->nx, req->ny, req->nz, req->prec};
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->checked.find(k);
+    if (it == s->checked.end()) {
+      gfft::PlanCore tmp;
+      it = s->checked.emplace(k, gfft::plan_host(tmp, req->nx, req->ny, req->nz, 1, 0, req->prec)).first;
+    }
+    if (it->second != GFFT_OK) return it->second;
+  }
   if (bytes > s->adm.budget()) {
     std::lock_guard<std::mutex> lk(s->mu);
     ++s->rejected;

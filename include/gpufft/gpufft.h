@@ -287,8 +287,9 @@ int gfft_server_create(gfft_server* srv, size_t budget_bytes, int nstreams, int 
  * which allocates and synchronises). */
 int gfft_server_prepare(gfft_server srv, int64_t nx, int64_t ny, int64_t nz, int prec);
 
-/* Queue a request.  Invalid geometry, GFFT_ERR_OVER_BUDGET (the array alone
- * exceeds the budget) and bad pointers are rejected here, nothing queued. */
+/* Queue a request.  Invalid geometry, sizes without a plan
+ * (GFFT_ERR_UNSUPPORTED_SIZE), GFFT_ERR_OVER_BUDGET (the array alone exceeds
+ * the budget) and bad pointers are rejected here, nothing queued. */
 int gfft_server_submit(gfft_server srv, const gfft_request* req, uint64_t* id);
 
 /* Non-blocking: move up to max finished results (completion order) to out. */
