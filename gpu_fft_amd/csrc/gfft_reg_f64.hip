@@ -17,8 +17,9 @@ int registry_f64(const KernelEntry** out) {
       GFFT_SIZE(double, Sc256, 8, 8, 8, 8),
       GFFT_SIZE(double, Sc384, 24, 4, 24, 8),
       // 512: measured default columns first (16 points per thread)
-      GFFT_ROWS(double, Sc512, 8, 4),
+      GFFT_ROWS(double, Sc512, 16, 4),
       GFFT_COLS(double, Sc512, 16, 8),
+      GFFT_ROWS(double, Sc512, 8, 4),
       GFFT_COLS(double, Sc512, 8, 8),
       GFFT_SIZE(double, Sc768, 24, 2, 24, 8),
       GFFT_SIZE(double, Sc1024, 8, 2, 16, 8),
@@ -31,7 +32,6 @@ int registry_f64(const KernelEntry** out) {
       GFFT_COLS(double, Sc512, 8, 4),
       GFFT_COLS(double, Sc512, 16, 4),
       GFFT_ROWS(double, Sc512, 8, 2),
-      GFFT_ROWS(double, Sc512, 16, 4),
       GFFT_COLS(double, Sc768, 24, 4),
       GFFT_COLS_PIPE(double, Sc512, 8, 8, 2, 0),
       GFFT_COLS_PIPE(double, Sc512, 16, 8, 2, 0),
