@@ -1,5 +1,6 @@
 """Large grids of the benchmark configurations (FP32): 1024^3 (4 GiB real),
-1024 x 512 x 2048 (K = 1025) and 768^3 (radix-3).  A full FP64 oracle at these
+1024 x 512 x 2048 (K = 1025) and 768^3 (radix-3), and the largest grid
+tested, 2048^3 (34 GB, more than 2^32 elements).  A full FP64 oracle at these
 sizes is minutes of CPU time, so parity uses size-independent checks:
 - a separable field f = a(x) b(y) c(z) whose spectrum is the closed form
   A(kx) B(ky) C(kz) (three 1D FP64 DFTs), for both directions;
@@ -29,7 +30,8 @@ def rel(torch, a, b):
     return float(torch.linalg.vector_norm((a - b).reshape(-1)) / torch.linalg.vector_norm(b.reshape(-1)))
 
 
-@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+# 2048^3 FP32: 34 GB, 8.6e9 reals (> 2^32): 64-bit offsets end to end
+@pytest.mark.parametrize("shape", SHAPES + [(2048, 2048, 2048)], ids=lambda s: "x".join(map(str, s)))
 def test_separable_closed_form(shape):
     import torch
     nx, ny, nz = shape
