@@ -36,7 +36,7 @@ const char* gfft_strerror(int status) {
     case GFFT_OK: return "success";
     case GFFT_ERR_INVALID_ARG: return "invalid argument (null handle or pointer, or bad precision flag)";
     case GFFT_ERR_INVALID_DIMS: return "invalid dimensions (an edge < 1, procs does not divide Nx and Ny, or overflow)";
-    case GFFT_ERR_UNSUPPORTED_SIZE: return "unsupported edge length (prime factor > 5 or above the LDS-resident limit)";
+    case GFFT_ERR_UNSUPPORTED_SIZE: return "unsupported edge length (prime factor > 13 or above the LDS-resident limit)";
     case GFFT_ERR_UNSUPPORTED_PROCS: return "full transform with procs > 1 needs an inter-rank exchange (not provided)";
     case GFFT_ERR_INVALID_RANK: return "rank outside [0, procs)";
     case GFFT_ERR_OUT_OF_MEMORY: return "device allocation failed";

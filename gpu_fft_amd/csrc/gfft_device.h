@@ -176,6 +176,112 @@ struct Dft<5, INV, T> {
   }
 };
 
+// cos(2 pi n / R) and sin(2 pi n / R) for n = 1..(R-1)/2 of the odd primes
+// 7, 11, 13 (20 significant digits, long double).
+template <int R>
+struct PrimeTab;
+template <>
+struct PrimeTab<7> {
+  __device__ __forceinline__ static constexpr double c(int n) {
+    return n == 1 ? 0.62348980185873355790 : n == 2 ? -0.22252093395631433606 : -0.90096886790241908068;
+  }
+  __device__ __forceinline__ static constexpr double s(int n) {
+    return n == 1 ? 0.78183148246802978688 : n == 2 ? 0.97492791218182362259 : 0.43388373911755821505;
+  }
+};
+template <>
+struct PrimeTab<11> {
+  __device__ __forceinline__ static constexpr double c(int n) {
+    return n == 1   ? 0.84125353283118118090
+           : n == 2 ? 0.41541501300188646601
+           : n == 3 ? -0.14231483827328507435
+           : n == 4 ? -0.65486073394528499680
+                    : -0.95949297361449735851;
+  }
+  __device__ __forceinline__ static constexpr double s(int n) {
+    return n == 1   ? 0.54064081745559756340
+           : n == 2 ? 0.90963199535451835293
+           : n == 3 ? 0.98982144188093274190
+           : n == 4 ? 0.75574957435425834209
+                    : 0.28173255684142980458;
+  }
+};
+template <>
+struct PrimeTab<13> {
+  __device__ __forceinline__ static constexpr double c(int n) {
+    return n == 1   ? 0.88545602565320990465
+           : n == 2 ? 0.56806474673115583350
+           : n == 3 ? 0.12053668025532310943
+           : n == 4 ? -0.35460488704253555553
+           : n == 5 ? -0.74851074817110103613
+                    : -0.97094181742605200014;
+  }
+  __device__ __forceinline__ static constexpr double s(int n) {
+    return n == 1   ? 0.46472317204376852899
+           : n == 2 ? 0.82298386589365637317
+           : n == 3 ? 0.99270887409805398600
+           : n == 4 ? 0.93501624268541485018
+           : n == 5 ? 0.66312265824079527296
+                    : 0.23931566428755787686;
+  }
+};
+
+// Odd prime R in the symmetric pair form (the radix-5 butterfly above is the
+// R = 5 case): with b_j = a_j + a_{R-j}, d_j = a_j - a_{R-j} (j = 1..m,
+// m = (R-1)/2),
+//   X_0 = a_0 + sum_j b_j,
+//   X_k, X_{R-k} = t_k +- W4 u_k,  t_k = a_0 + sum_j cos(2 pi jk/R) b_j,
+//                                  u_k = sum_j sin(2 pi jk/R) d_j,
+// W4 = -i forward, +i inverse.  (m+1)^2 real multiply-adds per component
+// instead of R^2.
+template <int R, bool INV, typename T>
+struct DftOdd {
+  static constexpr int M = (R - 1) / 2;
+  // cos / sin of 2 pi n / R for any n, folded into 1..m
+  __device__ __forceinline__ static constexpr T cs(int n) {
+    const int r = n % R;
+    return (T)PrimeTab<R>::c(r <= M ? r : R - r);
+  }
+  __device__ __forceinline__ static constexpr T sn(int n) {
+    const int r = n % R;
+    return r <= M ? (T)PrimeTab<R>::s(r) : (T)-PrimeTab<R>::s(R - r);
+  }
+  __device__ __forceinline__ static void run(cx<T>* a) {
+    cx<T> b[M], d[M];
+#pragma unroll
+    for (int j = 1; j <= M; ++j) {
+      b[j - 1] = a[j] + a[R - j];
+      d[j - 1] = a[j] - a[R - j];
+    }
+    const cx<T> a0 = a[0];
+    cx<T> y0 = a0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) y0 = y0 + b[j];
+#pragma unroll
+    for (int k = 1; k <= M; ++k) {
+      cx<T> t = a0;
+      cx<T> u = mk<T>((T)0, (T)0);
+#pragma unroll
+      for (int j = 1; j <= M; ++j) {
+        const T c = cs(j * k), s = sn(j * k);
+        t = mk<T>(fma(c, b[j - 1].x, t.x), fma(c, b[j - 1].y, t.y));
+        u = mk<T>(fma(s, d[j - 1].x, u.x), fma(s, d[j - 1].y, u.y));
+      }
+      u = rot4<INV>(u);
+      a[k] = t + u;
+      a[R - k] = t - u;
+    }
+    a[0] = y0;
+  }
+};
+
+template <bool INV, typename T>
+struct Dft<7, INV, T> : DftOdd<7, INV, T> {};
+template <bool INV, typename T>
+struct Dft<11, INV, T> : DftOdd<11, INV, T> {};
+template <bool INV, typename T>
+struct Dft<13, INV, T> : DftOdd<13, INV, T> {};
+
 template <bool INV, typename T>
 struct Dft<8, INV, T> {
   __device__ __forceinline__ static void run(cx<T>* a) {

@@ -1,6 +1,6 @@
 // Run-time-schedule kernels: the same Stockham recurrence and z-pass
 // two-for-one split as the register engine, with every stage staged through
-// LDS ping-pong buffers.  Used for any 2/3/5-smooth size that has no compiled
+// LDS ping-pong buffers.  Used for any 13-smooth size that has no compiled
 // register variant (the planner prefers compiled variants).
 #include "gfft_kernels.h"
 
@@ -47,6 +47,9 @@ __device__ __forceinline__ void gen_dispatch(int R, const cx<T>* in, cx<T>* out,
     case 3: gen_bfly<3, INV, T>(in, out, i, N, p, tw, ld); break;
     case 4: gen_bfly<4, INV, T>(in, out, i, N, p, tw, ld); break;
     case 5: gen_bfly<5, INV, T>(in, out, i, N, p, tw, ld); break;
+    case 7: gen_bfly<7, INV, T>(in, out, i, N, p, tw, ld); break;
+    case 11: gen_bfly<11, INV, T>(in, out, i, N, p, tw, ld); break;
+    case 13: gen_bfly<13, INV, T>(in, out, i, N, p, tw, ld); break;
     default: gen_bfly<8, INV, T>(in, out, i, N, p, tw, ld); break;
   }
 }

@@ -25,7 +25,7 @@ namespace gfft {
 
 namespace {
 
-bool factor_235(int64_t n, std::vector<int>& radices) {
+bool factor_smooth(int64_t n, std::vector<int>& radices) {
   radices.clear();
   if (n < 1) return false;
   int64_t m = n;
@@ -44,6 +44,12 @@ bool factor_235(int64_t n, std::vector<int>& radices) {
     m /= 5;
     ++c5;
   }
+  std::vector<int> big;  // 7, 11, 13 go last
+  for (int q : {7, 11, 13})
+    while (m % q == 0) {
+      m /= q;
+      big.push_back(q);
+    }
   if (m != 1) return false;
   while (c2 >= 3) {
     radices.push_back(8);
@@ -53,6 +59,7 @@ bool factor_235(int64_t n, std::vector<int>& radices) {
   if (c2 == 1) radices.push_back(2);
   for (int i = 0; i < c5; ++i) radices.push_back(5);
   for (int i = 0; i < c3; ++i) radices.push_back(3);
+  radices.insert(radices.end(), big.begin(), big.end());
   return true;
 }
 
@@ -156,7 +163,7 @@ int plan_axis(AxisPlan& a, int prec, int64_t n, bool rows, int64_t K) {
   a.active = rows || n > 1;
   if (!a.active) return GFFT_OK;
   std::vector<int> rad;
-  if (n > (1 << 20) || !factor_235(n, rad)) return GFFT_ERR_UNSUPPORTED_SIZE;
+  if (n > (1 << 20) || !factor_smooth(n, rad)) return GFFT_ERR_UNSUPPORTED_SIZE;
   const int kf = rows ? ROWS_R2C : COLS_FWD;
   const int ki = rows ? ROWS_C2R : COLS_INV;
   a.cands = find_pairs(prec, kf, ki, (int)n);

@@ -376,7 +376,7 @@ int gfft_server_submit(gfft_server s, const gfft_request* req, uint64_t* id) {
   int st = request_bytes(req->nx, req->ny, req->nz, req->prec, &bytes);
   if (st != GFFT_OK) return st;
   {
-    // sizes the planner cannot serve (a prime factor > 5, ...) fail here,
+    // sizes the planner cannot serve (a prime factor > 13, ...) fail here,
     // not at admission; the host-only check is cached per geometry
     const Key k{req->nx, req->ny, req->nz, req->prec};
     std::lock_guard<std::mutex> lk(s->mu);

@@ -44,8 +44,10 @@ def test_strerror_covers_codes():
     ((12, 8, 8, 8, 0, 0), 2),    # procs does not divide Nx
     ((8, 12, 8, 8, 0, 0), 2),    # ... nor Ny
     ((8, 8, 8, 1, 0, 7), 1),     # bad precision flag
-    ((8, 8, 14, 1, 0, 0), 3),    # prime factor 7
-    ((11, 8, 8, 1, 0, 1), 3),    # prime factor 11
+    ((8, 8, 34, 1, 0, 0), 3),    # prime factor 17
+    ((19, 8, 8, 1, 0, 1), 3),    # prime factor 19
+    ((8, 8, 14, 1, 0, 0), 0),    # 7, 11, 13 are radices
+    ((11, 13, 8, 1, 0, 1), 0),
     ((8, 8, 8, 1, 0, 0), 0),
     ((768, 768, 768, 1, 0, 0), 0),
     ((128, 1024, 1024, 8, 3, 0), 0),
@@ -122,3 +124,11 @@ def test_all_235_smooth_sizes_plan(n):
         for v in r:
             prod *= v
         assert prod == n and set(r) <= {2, 3, 4, 5, 8}
+
+
+@pytest.mark.parametrize("n", [7, 11, 13, 14, 49, 77, 91, 143, 169, 343, 448, 1001, 1183])
+def test_13_smooth_sizes_plan(n):
+    d = G.query_plan(n, n, n, precision="f64")
+    for ax in ("z", "y", "x"):
+        r = d[ax]["radices"]
+        assert math.prod(r) == n and set(r) <= {2, 3, 4, 5, 7, 8, 11, 13}

@@ -132,8 +132,8 @@ def test_over_budget_and_bad_requests_rejected():
         with pytest.raises(G.GfftError) as e:
             srv.submit((64, 64, 0), "f32", S.R2C, x)
         assert e.value.status == 2
-        with pytest.raises(G.GfftError) as e:  # prime factor 7: no plan
-            srv.submit((7, 7, 14), "f32", S.R2C, x)
+        with pytest.raises(G.GfftError) as e:  # prime factor 17: no plan
+            srv.submit((17, 17, 34), "f32", S.R2C, x)
         assert e.value.status == 3
         host = torch.zeros(8, 8, 10)
         with pytest.raises(G.GfftError) as e:
