@@ -86,6 +86,7 @@ struct KernelEntry {
   int persist;      // 0: one tile per workgroup; k: persistent, k workgroups per CU
   int order;        // tile order: persistent 0 contiguous ranges, 1 round robin; one tile per
                     // workgroup 0 XCD-aware ranges, 2 linear
+  int readdr = 0;   // persistent columns: 1 = element offsets recomputed per tile (not hoisted)
 };
 
 // Registry units of compiled variants; each returns its number of entries.

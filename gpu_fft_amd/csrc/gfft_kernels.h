@@ -372,7 +372,13 @@ constexpr int twiddle_count() {
   return SC::twoff(SC::S) > 0 ? SC::twoff(SC::S) : 1;
 }
 
-template <typename T, bool INV, typename SC, int P, int CW, int ORD>
+// RA 1: the thread's column and row indexes are made opaque to the compiler
+// at the top of every tile, so the per-element global and LDS offsets
+// derived from them are recomputed per tile.  Otherwise (RA 0) they are
+// hoisted out of the tile loop and held in registers for the whole kernel:
+// cheaper for small tiles, but for 768/1024-point columns at 16-24 points
+// per thread that exceeds the register budget (spills, or half the waves).
+template <typename T, bool INV, typename SC, int P, int CW, int ORD, int RA = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPipe<T, INV, SC, P, CW>::NT))) void k_cols_pipe(
     const cx<T>* in, cx<T>* out, ColGeom g, const cx<T>* tw, uint32_t ntiles) {
   using K_ = ColsPipe<T, INV, SC, P, CW>;
@@ -381,11 +387,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPipe<T, INV, SC, P,
   const TileWalk walk = tile_walk<ORD>(ntiles, g.rev);
   if (walk.count == 0) return;  // uniform across the workgroup
   copy_twiddles<SC>(stw, tw, K_::NT);
-  const int c = threadIdx.x % CW, t = threadIdx.x / CW;
-  ColLds<T, CW> L{smem, c};
+  const int c0 = threadIdx.x % CW, t0 = threadIdx.x / CW;
   cx<T> cur[K_::E::PV], nxt[K_::E::PV];
-  K_::load(nxt, in, g, walk.at(0), c, t);
+  K_::load(nxt, in, g, walk.at(0), c0, t0);
   for (uint32_t k = 0; k < walk.count; ++k) {
+    int c = c0, t = t0;
+    if constexpr (RA != 0) __asm__ volatile("" : "+v"(c), "+v"(t));
+    ColLds<T, CW> L{smem, c};
 #pragma unroll
     for (int i = 0; i < K_::E::PV; ++i) cur[i] = nxt[i];
     if (k + 1 < walk.count) K_::load(nxt, in, g, walk.at(k + 1), c, t);
@@ -486,11 +494,11 @@ inline int64_t persistent_blocks(int64_t ntiles, int occ) {
   return blocks < ntiles ? blocks : ntiles;
 }
 
-template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD>
+template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD, int RA = 0>
 hipError_t launch_cols_pipe(const void* in, void* out, ColGeom g, const void* tw, hipStream_t s) {
   const int64_t ntiles = ((int64_t)g.ncols + CW - 1) / CW;
   if (ntiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_cols_pipe<T, INV, SC, P, CW, ORD>), dim3((uint32_t)persistent_blocks(ntiles, OCC)),
+  hipLaunchKernelGGL((k_cols_pipe<T, INV, SC, P, CW, ORD, RA>), dim3((uint32_t)persistent_blocks(ntiles, OCC)),
                      dim3(ColsPipe<T, INV, SC, P, CW>::NT), 0, s, static_cast<const cx<T>*>(in),
                      static_cast<cx<T>*>(out), g,
                      static_cast<const cx<T>*>(tw), (uint32_t)ntiles);
@@ -549,7 +557,7 @@ KernelEntry entry_cols() {
                      reinterpret_cast<void*>(&launch_cols<T, INV, SC, P, CW, ORD>), 0, ORD};
 }
 
-template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD>
+template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD, int RA = 0>
 KernelEntry entry_cols_pipe() {
   return KernelEntry{prec_of<T>(),
                      INV ? COLS_INV : COLS_FWD,
@@ -558,9 +566,10 @@ KernelEntry entry_cols_pipe() {
                      P,
                      CW,
                      ColsPipe<T, INV, SC, P, CW>::NT,
-                     reinterpret_cast<void*>(&launch_cols_pipe<T, INV, SC, P, CW, OCC, ORD>),
+                     reinterpret_cast<void*>(&launch_cols_pipe<T, INV, SC, P, CW, OCC, ORD, RA>),
                      OCC,
-                     ORD};
+                     ORD,
+                     RA};
 }
 
 template <typename T, typename SC, int P, int G, int OCC, int ORD>

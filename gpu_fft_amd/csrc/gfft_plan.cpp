@@ -104,7 +104,7 @@ std::vector<std::pair<const KernelEntry*, const KernelEntry*>> find_pairs(int pr
     if (tab[i].kind != kf || tab[i].N != n) continue;
     for (int j = 0; j < cnt; ++j)
       if (tab[j].kind == ki && tab[j].N == n && tab[j].P == tab[i].P && tab[j].tile == tab[i].tile &&
-          tab[j].persist == tab[i].persist && tab[j].order == tab[i].order &&
+          tab[j].persist == tab[i].persist && tab[j].order == tab[i].order && tab[j].readdr == tab[i].readdr &&
           same_sched(tab[j].sched, tab[i].sched)) {
         out.emplace_back(&tab[i], &tab[j]);
         break;
@@ -216,10 +216,11 @@ std::string axis_json(const char* name, const AxisPlan& a) {
   std::snprintf(buf, sizeof buf,
                 "\"%s\":{\"n\":%d,\"pass\":\"%s\",\"engine\":\"%s\",\"radices\":%s,\"points_per_thread\":%d,"
                 "\"tile\":%d,\"threads\":%d,\"twiddle_bytes\":%zu,\"variant\":%d,\"candidates\":%d,"
-                "\"persist\":%d,\"order\":%d,\"tuned_ms\":%.6f}",
+                "\"persist\":%d,\"order\":%d,\"readdr\":%d,\"tuned_ms\":%.6f}",
                 name, a.N, a.rows ? "rows_two_for_one" : "strided_columns", e ? "register" : "generic",
                 sched_str(a.sched).c_str(), e ? e->P : 0, e ? e->tile : 0, e ? e->threads : 256, a.tw_bytes,
-                a.variant, (int)a.cands.size(), e ? e->persist : 0, e ? e->order : 0, a.tuned_ms);
+                a.variant, (int)a.cands.size(), e ? e->persist : 0, e ? e->order : 0, e ? e->readdr : 0,
+                a.tuned_ms);
   return buf;
 }
 

@@ -22,10 +22,12 @@ int registry_f32(const KernelEntry** out) {
       GFFT_COLS_PIPE(float, Sc512, 16, 16, 1, 1),
       GFFT_SIZE(float, Sc512, 8, 4, 8, 16),
       // 768, 1024: measured defaults first (z in the radix order 8,4,8,3 /
-      // 8,4,4,8; strided passes non-persistent, 16 / 8 columns per tile)
+      // 8,4,4,8; strided passes persistent, pipelined, 16 columns per tile,
+      // round-robin tiles, element offsets recomputed per tile)
       GFFT_ROWS(float, Sc768c, 24, 4),
-      GFFT_COLS(float, Sc768, 24, 16),
+      GFFT_COLS_PIPE_RA(float, Sc768, 24, 16, 1, 1),
       GFFT_ROWS(float, Sc1024c, 8, 2),
+      GFFT_COLS_PIPE_RA(float, Sc1024, 16, 16, 1, 1),
       GFFT_COLS(float, Sc1024, 16, 8),
       GFFT_SIZE(float, Sc768, 24, 4, 24, 16),
       GFFT_SIZE(float, Sc1024, 8, 2, 16, 16),
@@ -48,13 +50,13 @@ int registry_f32(const KernelEntry** out) {
       GFFT_COLS(float, Sc768, 24, 8),
       GFFT_COLS_PIPE(float, Sc1024, 16, 8, 2, 0),
       GFFT_COLS_PIPE(float, Sc1024, 16, 8, 2, 1),
-      GFFT_COLS_PIPE(float, Sc1024, 16, 16, 1, 0),
-      GFFT_COLS_PIPE(float, Sc1024, 16, 16, 1, 1),
+      GFFT_COLS_PIPE_RA(float, Sc1024, 16, 16, 1, 0),
+      GFFT_COLS_PIPE_RA(float, Sc1024, 32, 16, 1, 0),
+      GFFT_COLS_PIPE_RA(float, Sc1024, 32, 16, 1, 1),
       GFFT_COLS_PIPE(float, Sc768, 24, 8, 2, 0),
       GFFT_COLS_PIPE(float, Sc768, 24, 8, 2, 1),
-      GFFT_COLS_PIPE(float, Sc768, 24, 16, 1, 0),
-      GFFT_COLS_PIPE(float, Sc768, 24, 16, 1, 1),
-      GFFT_COLS_PIPE(float, Sc256, 8, 16, 2, 0),
+      GFFT_COLS_PIPE_RA(float, Sc768, 24, 16, 1, 0),
+            GFFT_COLS_PIPE(float, Sc256, 8, 16, 2, 0),
       GFFT_COLS_PIPE(float, Sc256, 8, 16, 2, 1),
       GFFT_COLS_PIPE(float, Sc512, 32, 32, 1, 0),
       GFFT_COLS_PIPE(float, Sc512, 16, 32, 1, 0),
@@ -72,11 +74,11 @@ int registry_f32(const KernelEntry** out) {
       GFFT_COLS_LIN(float, Sc256, 8, 16),
       // other radix orders
       GFFT_ROWS(float, Sc1024b, 8, 2),
-      GFFT_COLS_PIPE(float, Sc1024b, 16, 16, 1, 0),
-      GFFT_COLS_PIPE(float, Sc1024c, 16, 16, 1, 0),
+      GFFT_COLS_PIPE_RA(float, Sc1024b, 16, 16, 1, 0),
+      GFFT_COLS_PIPE_RA(float, Sc1024c, 16, 16, 1, 0),
       GFFT_ROWS(float, Sc768b, 24, 4),
-      GFFT_COLS_PIPE(float, Sc768b, 24, 16, 1, 0),
-      GFFT_COLS_PIPE(float, Sc768c, 24, 16, 1, 0),
+      GFFT_COLS_PIPE_RA(float, Sc768b, 24, 16, 1, 0),
+      GFFT_COLS_PIPE_RA(float, Sc768c, 24, 16, 1, 0),
       GFFT_COLS_PIPE(float, Sc512, 8, 16, 2, 0),
       GFFT_COLS_PIPE(float, Sc512, 16, 16, 2, 0),
       GFFT_COLS_PIPE(float, Sc512, 16, 16, 1, 0),

@@ -75,6 +75,9 @@ using Sc500 = Sched<4, 5, 5, 5>;
 // persistent software-pipelined column pass, OCC workgroups per CU
 #define GFFT_COLS_PIPE(T, SC, PC, CW, OCC, ORD) \
   entry_cols_pipe<T, false, SC, PC, CW, OCC, ORD>(), entry_cols_pipe<T, true, SC, PC, CW, OCC, ORD>()
+// the same with per-tile recomputed element offsets (RA 1 in k_cols_pipe)
+#define GFFT_COLS_PIPE_RA(T, SC, PC, CW, OCC, ORD) \
+  entry_cols_pipe<T, false, SC, PC, CW, OCC, ORD, 1>(), entry_cols_pipe<T, true, SC, PC, CW, OCC, ORD, 1>()
 // persistent software-pipelined z passes, G row pairs per tile
 #define GFFT_ROWS_PIPE(T, SC, PR, G, OCC, ORD) \
   entry_rows_r2c_pipe<T, SC, PR, G, OCC, ORD>(), entry_rows_c2r_pipe<T, SC, PR, G, OCC, ORD>()
