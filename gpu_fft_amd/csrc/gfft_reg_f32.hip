@@ -21,16 +21,18 @@ int registry_f32(const KernelEntry** out) {
       GFFT_ROWS(float, Sc512, 8, 8),
       GFFT_COLS_PIPE(float, Sc512, 16, 16, 1, 1),
       GFFT_SIZE(float, Sc512, 8, 4, 8, 16),
-      // 768, 1024: measured defaults first (z in the radix order 8,4,8,3 /
-      // 8,4,4,8; strided passes persistent, pipelined, 16 columns per tile,
-      // round-robin tiles, element offsets recomputed per tile)
+      // 768, 1024, 2048: measured defaults first (z in the radix order 8,4,8,3 /
+      // 8,4,4,8 / 8,8,8,4; 4 / 4 / 2 row pairs per tile; strided passes
+      // persistent, pipelined, 16 columns per tile, round-robin tiles,
+      // element offsets recomputed per tile)
       GFFT_ROWS(float, Sc768c, 24, 4),
       GFFT_COLS_PIPE_RA(float, Sc768, 24, 16, 1, 1),
-      GFFT_ROWS(float, Sc1024c, 8, 2),
+      GFFT_ROWS(float, Sc1024c, 8, 4),
       GFFT_COLS_PIPE_RA(float, Sc1024, 16, 16, 1, 1),
       GFFT_COLS(float, Sc1024, 16, 8),
       GFFT_SIZE(float, Sc768, 24, 4, 24, 16),
       GFFT_SIZE(float, Sc1024, 8, 2, 16, 16),
+      GFFT_ROWS(float, Sc2048, 16, 2),
       GFFT_SIZE(float, Sc2048, 8, 1, 16, 8),
       // tuning alternatives
       GFFT_SIZE(float, Sc160, 40, 32, 40, 16),
@@ -72,6 +74,18 @@ int registry_f32(const KernelEntry** out) {
       GFFT_COLS_LIN(float, Sc1024, 16, 16),
       GFFT_COLS_LIN(float, Sc1024, 32, 16),
       GFFT_COLS_LIN(float, Sc256, 8, 16),
+      // z-pass alternatives for the long rows (768, 1024, 2048)
+      GFFT_ROWS(float, Sc1024c, 16, 2),
+      GFFT_ROWS(float, Sc1024c, 8, 2),
+      GFFT_ROWS(float, Sc1024c, 16, 4),
+      GFFT_ROWS_PIPE(float, Sc1024c, 8, 2, 4, 1),
+      GFFT_ROWS_PIPE(float, Sc1024c, 16, 4, 2, 1),
+      GFFT_ROWS(float, Sc2048, 16, 1),
+      GFFT_ROWS(float, Sc2048, 8, 2),
+      GFFT_ROWS_PIPE(float, Sc2048, 16, 2, 2, 1),
+      GFFT_ROWS(float, Sc768c, 24, 8),
+      GFFT_ROWS(float, Sc768c, 16, 4),
+      GFFT_ROWS_PIPE(float, Sc768c, 24, 8, 2, 1),
       // other radix orders
       GFFT_ROWS(float, Sc1024b, 8, 2),
       GFFT_COLS_PIPE_RA(float, Sc1024b, 16, 16, 1, 0),
