@@ -484,7 +484,7 @@ struct RegEngine {
 };
 
 // ---------------------------------------------------------------------------
-// Run-time schedule (any product of 2, 3, 4, 5, 8)
+// Run-time schedule (any product of 2, 3, 4, 5, 7, 8, 11, 13)
 // ---------------------------------------------------------------------------
 constexpr int kMaxStages = 16;
 struct RtSched {
