@@ -1,7 +1,7 @@
 // Device-side building blocks of the gfx950 FFT engine.
 //
 //  * cx<T>            : POD complex, 8/16-byte aligned (one dwordx2 / dwordx4)
-//  * dft<R, INV>      : radix-2/3/4/5/8 butterflies, FMA form, in registers
+//  * Dft<R, INV>      : radix-2/3/4/5/7/8/11/13 butterflies, FMA form, in registers
 //  * Sched<R...>      : a compile-time mixed-radix Stockham schedule
 //  * RegEngine        : register-resident Stockham; each thread owns P points,
 //                       stages exchange through LDS (one barrier pair each)
