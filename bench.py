@@ -494,8 +494,14 @@ def main():
     desc = plan.describe()
     log("plan", json.dumps(desc))
 
-    passes = [(0, False), (1, False), (2, False), (2, True), (1, True), (0, True)]
-    names = ["z_r2c", "y_fwd", "x_fwd", "x_inv", "y_inv", "z_c2r"]
+    if desc.get("zy_fused"):
+        # small planes: z and y run as one fused LDS-resident plane pass
+        # (axis 3); its HBM bytes per launch are 2 S like every other pass
+        passes = [(3, False), (2, False), (2, True), (3, True)]
+        names = ["zy_r2c", "x_fwd", "x_inv", "zy_c2r"]
+    else:
+        passes = [(0, False), (1, False), (2, False), (2, True), (1, True), (0, True)]
+        names = ["z_r2c", "y_fwd", "x_fwd", "x_inv", "y_inv", "z_c2r"]
 
     def step(ev=None):
         for j, (ax, inv) in enumerate(passes):
@@ -570,6 +576,7 @@ def main():
         "per_pass": per_pass,
         "variants": {ax: {k: desc[ax][k] for k in ("engine", "radices", "points_per_thread", "tile", "threads",
                                                    "persist", "order", "readdr", "variant")} for ax in ("z", "y", "x")},
+        "zy_fused": int(desc.get("zy_fused", 0)),
         "wall_ms_per_step": round(wall / args.steps * 1e3, 5),
         "output_rms": round(rms, 5),
         "output_finite": finite,

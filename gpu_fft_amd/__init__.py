@@ -74,6 +74,8 @@ def lib() -> ctypes.CDLL:
     L.gfft_plan_tune.argtypes = [vp, vp, vp, ctypes.c_int]
     L.gfft_plan_set_variant.restype = ctypes.c_int
     L.gfft_plan_set_variant.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.gfft_plan_set_fused.restype = ctypes.c_int
+    L.gfft_plan_set_fused.argtypes = [vp, ctypes.c_int]
     L.gfft_plan_set_chunking.restype = ctypes.c_int
     L.gfft_plan_set_chunking.argtypes = [vp, ctypes.c_int64]
     L.gfft_plan_set_walk.restype = ctypes.c_int
@@ -197,6 +199,10 @@ class Plan:
     def set_variant(self, axis: int, variant: int) -> None:
         _check(lib().gfft_plan_set_variant(self.handle, int(axis), int(variant)), "gfft_plan_set_variant")
 
+    def set_fused(self, on: bool) -> None:
+        """Use (True) or bypass (False) the fused z + y plane kernels."""
+        _check(lib().gfft_plan_set_fused(self.handle, 1 if on else 0), "gfft_plan_set_fused")
+
     def set_chunking(self, planes: int) -> None:
         """Alternate z and y passes per block of ``planes`` x-planes (0 = off)."""
         _check(lib().gfft_plan_set_chunking(self.handle, int(planes)), "gfft_plan_set_chunking")
@@ -211,7 +217,8 @@ class Plan:
         _check(lib().gfft_plan_set_scale(self.handle, float(r2c_scale), float(c2r_scale)), "gfft_plan_set_scale")
 
     def axis(self, axis: int, inverse: bool, data, stream=None) -> None:
-        """One in-place axis pass (0 = z, 1 = y, 2 = x)."""
+        """One in-place axis pass (0 = z, 1 = y, 2 = x; 3 = the fused z + y
+        plane pass when the plan uses it)."""
         _check(lib().gfft_exec_axis(self.handle, int(axis), int(bool(inverse)), _ptr(data), _stream_handle(stream)),
                "gfft_exec_axis")
 

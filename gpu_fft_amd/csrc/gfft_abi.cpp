@@ -97,7 +97,7 @@ int gfft_exec_c2r(gfft_plan plan, void* data, gfft_stream_t stream) {
 }
 
 int gfft_exec_axis(gfft_plan plan, int axis, int inverse, void* data, gfft_stream_t stream) {
-  if (!plan || !data || axis < 0 || axis > 2) return GFFT_ERR_INVALID_ARG;
+  if (!plan || !data || axis < 0 || axis > 3) return GFFT_ERR_INVALID_ARG;
   const int st = check_device(plan);
   if (st != GFFT_OK) return st;
   return gfft::exec_axis(plan->core, axis, inverse != 0, data, reinterpret_cast<hipStream_t>(stream));
@@ -149,6 +149,11 @@ int gfft_plan_set_chunking(gfft_plan plan, int64_t planes) {
   return GFFT_OK;
 }
 
+int gfft_plan_set_fused(gfft_plan plan, int on) {
+  if (!plan) return GFFT_ERR_INVALID_ARG;
+  return gfft::set_fused(plan->core, on);
+}
+
 int gfft_plan_set_walk(gfft_plan plan, int mode) {
   if (!plan || mode < 0 || mode > 1) return GFFT_ERR_INVALID_ARG;
   plan->core.walk = mode;
@@ -164,7 +169,7 @@ int gfft_plan_set_scale(gfft_plan plan, double r2c_scale, double c2r_scale) {
 
 int gfft_plan_device_bytes(gfft_plan plan, size_t* bytes) {
   if (!plan || !bytes) return GFFT_ERR_INVALID_ARG;
-  *bytes = plan->core.z.tw_bytes + plan->core.y.tw_bytes + plan->core.x.tw_bytes;
+  *bytes = gfft::plan_device_bytes(plan->core);
   return GFFT_OK;
 }
 

@@ -89,6 +89,19 @@ struct KernelEntry {
   int readdr = 0;   // persistent columns: 1 = element offsets recomputed per tile (not hoisted)
 };
 
+// Fused z + y kernel pair for x-planes that fit in LDS (gfft_plane.h).
+struct PlaneEntry {
+  int prec;
+  int nz, ny;
+  Schedule schedz, schedy;  // radix orders of the z and y engines
+  int threads;
+  void* fn_r2c;  // PlaneLaunchFn
+  void* fn_c2r;
+};
+typedef hipError_t (*PlaneLaunchFn)(void* data, int64_t nplanes, double scale, const void* twz, const void* twy,
+                                    hipStream_t s);
+int registry_plane(const PlaneEntry** out);
+
 // Registry units of compiled variants; each returns its number of entries.
 int registry_f32(const KernelEntry** out);
 int registry_f64(const KernelEntry** out);

@@ -266,6 +266,13 @@ int plan_host(PlanCore& c, int64_t nx, int64_t ny, int64_t nz, int procs, int ra
   if ((st = plan_axis(c.z, prec, nz, true, c.K)) != GFFT_OK) return st;
   if ((st = plan_axis(c.y, prec, ny, false, c.K)) != GFFT_OK) return st;
   if ((st = plan_axis(c.x, prec, nx, false, c.K)) != GFFT_OK) return st;
+  if (procs == 1 && c.y.active) {
+    const PlaneEntry* tab = nullptr;
+    const int cnt = registry_plane(&tab);
+    for (int i = 0; i < cnt && !c.plane; ++i)
+      if (tab[i].prec == prec && tab[i].nz == nz && tab[i].ny == ny) c.plane = &tab[i];
+    c.fused = c.plane ? 1 : 0;
+  }
   const size_t cs = prec == F32 ? 8 : 16;
   for (AxisPlan* a : {&c.z, &c.y, &c.x}) {
     if (!a->active) continue;
@@ -275,7 +282,17 @@ int plan_host(PlanCore& c, int64_t nx, int64_t ny, int64_t nz, int procs, int ra
       a->tw_bytes += n / 2 * cs;
     }
   }
+  if (c.plane) {
+    for (const Schedule* sc : {&c.plane->schedz, &c.plane->schedy}) {
+      const size_t n = prec == F32 ? make_twiddles<float>(*sc).size() : make_twiddles<double>(*sc).size();
+      c.plane_tw_bytes += n / 2 * cs;
+    }
+  }
   return GFFT_OK;
+}
+
+size_t plan_device_bytes(const PlanCore& c) {
+  return c.z.tw_bytes + c.y.tw_bytes + c.x.tw_bytes + c.plane_tw_bytes;
 }
 
 std::string plan_json(const PlanCore& c) {
@@ -284,7 +301,10 @@ std::string plan_json(const PlanCore& c) {
                 "{\"nx\":%lld,\"ny\":%lld,\"nz\":%lld,\"procs\":%d,\"rank\":%d,\"precision\":\"%s\",\"K\":%lld,",
                 (long long)c.nx, (long long)c.ny, (long long)c.nz, c.procs, c.rank, c.prec == F32 ? "f32" : "f64",
                 (long long)c.K);
-  return std::string(head) + axis_json("z", c.z) + "," + axis_json("y", c.y) + "," + axis_json("x", c.x) + "}";
+  char tail[160];
+  std::snprintf(tail, sizeof tail, ",\"zy_plane_kernel\":%d,\"zy_fused\":%d,\"fused_ms\":%.6f,\"unfused_ms\":%.6f}",
+                c.plane ? 1 : 0, (c.plane && c.fused) ? 1 : 0, c.fused_ms, c.unfused_ms);
+  return std::string(head) + axis_json("z", c.z) + "," + axis_json("y", c.y) + "," + axis_json("x", c.x) + tail;
 }
 
 int upload_twiddles(PlanCore& c) {
@@ -304,6 +324,20 @@ int upload_twiddles(PlanCore& c) {
     }
     a->d_tw = a->d_tws[a->cand_sched[a->variant]];
   }
+  if (c.plane) {
+    c.plane_tw_bytes = 0;
+    for (int i = 0; i < 2; ++i) {
+      const std::vector<unsigned char> t = twiddle_table(i == 0 ? c.plane->schedz : c.plane->schedy, c.prec);
+      void*& d = i == 0 ? c.d_twz_plane : c.d_twy_plane;
+      if (hipMalloc(&d, t.size()) != hipSuccess) {
+        d = nullptr;
+        (void)hipGetLastError();
+        return GFFT_ERR_OUT_OF_MEMORY;
+      }
+      c.plane_tw_bytes += t.size();
+      if (hipMemcpy(d, t.data(), t.size(), hipMemcpyHostToDevice) != hipSuccess) return GFFT_ERR_HIP;
+    }
+  }
   return GFFT_OK;
 }
 
@@ -313,6 +347,10 @@ void free_twiddles(PlanCore& c) {
       if (d) (void)hipFree(d);
     a->d_tws.clear();
     a->d_tw = nullptr;
+  }
+  for (void** d : {&c.d_twz_plane, &c.d_twy_plane}) {
+    if (*d) (void)hipFree(*d);
+    *d = nullptr;
   }
 }
 
@@ -424,6 +462,14 @@ bool reversed_walk(const PlanCore& c, int axis, bool inverse) {
   return c.walk == 1 && (inverse ? axis != 1 : axis == 1);
 }
 
+bool use_plane(const PlanCore& c) { return c.fused && c.plane && c.d_twz_plane && c.d_twy_plane; }
+
+hipError_t run_plane(const PlanCore& c, bool inverse, void* data, hipStream_t s) {
+  const PlaneLaunchFn f = reinterpret_cast<PlaneLaunchFn>(inverse ? c.plane->fn_c2r : c.plane->fn_r2c);
+  return f(data, c.nx, inverse ? axis_scale(c, 0, true) : axis_scale(c, 1, false), c.d_twz_plane, c.d_twy_plane,
+           s);
+}
+
 int exec_axis(const PlanCore& c, int axis, bool inverse, void* data, hipStream_t s) {
   hipError_t e = hipSuccess;
   const double sc = axis_scale(c, axis, inverse);
@@ -447,6 +493,11 @@ int exec_axis(const PlanCore& c, int axis, bool inverse, void* data, hipStream_t
       e = run_cols(c, c.x, inverse, data, g, s);
       break;
     }
+    case 3: {
+      if (!use_plane(c)) return GFFT_ERR_INVALID_ARG;
+      e = run_plane(c, inverse, data, s);
+      break;
+    }
     default: return GFFT_ERR_INVALID_ARG;
   }
   return e == hipSuccess ? GFFT_OK : GFFT_ERR_HIP;
@@ -465,6 +516,12 @@ int set_variant(PlanCore& c, int axis, int variant) {
   a->sched = a->scheds[a->cand_sched[variant]];
   if (!a->d_tws.empty()) a->d_tw = a->d_tws[a->cand_sched[variant]];
   a->tuned_ms = -1;
+  return GFFT_OK;
+}
+
+int set_fused(PlanCore& c, int on) {
+  if (on && !c.plane) return GFFT_ERR_INVALID_ARG;
+  c.fused = on ? 1 : 0;
   return GFFT_OK;
 }
 
@@ -541,6 +598,50 @@ int tune(PlanCore& c, void* scratch, hipStream_t s, int reps) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = best;
   }
+  // fused z + y plane kernels against the (tuned) separate passes
+  if (st == GFFT_OK && c.plane && c.d_twz_plane) {
+    char kb[128];
+    std::snprintf(kb, sizeof kb, "p%d:plane:%lld:%lld:%lld", c.prec, (long long)c.nx, (long long)c.ny,
+                  (long long)c.nz);
+    int cached = -1;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune_cache.find(kb);
+      if (it != g_tune_cache.end()) cached = it->second;
+    }
+    if (cached >= 0) {
+      c.fused = cached;
+    } else {
+      float ms[2] = {0, 0};
+      for (int mode = 0; mode < 2 && st == GFFT_OK; ++mode) {
+        // whole round trips (the x pass included: its launch overlaps
+        // differently with one or two z/y launches before it)
+        c.fused = mode;
+        auto once = [&]() -> int {
+          const int r = exec_r2c(c, scratch, s);
+          return r == GFFT_OK ? exec_c2r(c, scratch, s) : r;
+        };
+        if ((st = once()) != GFFT_OK) break;
+        if (hipEventRecord(e0, s) != hipSuccess) st = GFFT_ERR_HIP;
+        for (int r = 0; r < reps && st == GFFT_OK; ++r) st = once();
+        if (st != GFFT_OK) break;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms[mode], e0, e1) != hipSuccess) {
+          st = GFFT_ERR_HIP;
+          break;
+        }
+        ms[mode] /= reps;
+      }
+      c.fused = 1;
+      if (st == GFFT_OK) {
+        c.unfused_ms = ms[0];
+        c.fused_ms = ms[1];
+        c.fused = ms[1] < ms[0] ? 1 : 0;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[kb] = c.fused;
+      }
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return st;
@@ -575,6 +676,10 @@ int exec_zy_chunked(const PlanCore& c, bool inverse, void* data, hipStream_t s) 
 bool use_chunks(const PlanCore& c) { return c.chunk_planes > 0 && c.chunk_planes < c.nx / c.procs; }
 
 int exec_r2c(const PlanCore& c, void* data, hipStream_t s) {
+  if (use_plane(c)) {
+    if (run_plane(c, false, data, s) != hipSuccess) return GFFT_ERR_HIP;
+    return exec_axis(c, 2, false, data, s);
+  }
   if (use_chunks(c)) {
     const int st = exec_zy_chunked(c, false, data, s);
     if (st != GFFT_OK) return st;
@@ -588,6 +693,11 @@ int exec_r2c(const PlanCore& c, void* data, hipStream_t s) {
 }
 
 int exec_c2r(const PlanCore& c, void* data, hipStream_t s) {
+  if (use_plane(c)) {
+    const int st = exec_axis(c, 2, true, data, s);
+    if (st != GFFT_OK) return st;
+    return run_plane(c, true, data, s) == hipSuccess ? GFFT_OK : GFFT_ERR_HIP;
+  }
   if (use_chunks(c)) {
     const int st = exec_axis(c, 2, true, data, s);
     if (st != GFFT_OK) return st;

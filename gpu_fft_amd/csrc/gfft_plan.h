@@ -38,6 +38,14 @@ struct PlanCore {
   int64_t chunk_planes = 0;  // > 0: z and y passes alternate per block of x-planes
   int walk = 1;              // 1: passes alternate their tile order (see exec_axis)
   AxisPlan z, y, x;
+  // fused z + y kernels over LDS-resident x-planes (null: none compiled for
+  // this geometry); `fused` selects them for full transforms
+  const PlaneEntry* plane = nullptr;
+  void* d_twz_plane = nullptr;
+  void* d_twy_plane = nullptr;
+  size_t plane_tw_bytes = 0;
+  int fused = 0;
+  double fused_ms = -1, unfused_ms = -1;  // z + y round-trip times when tuned
 };
 
 int validate_dims(int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int prec);
@@ -49,6 +57,8 @@ int exec_r2c(const PlanCore& c, void* data, hipStream_t s);
 int exec_c2r(const PlanCore& c, void* data, hipStream_t s);
 int exec_axis(const PlanCore& c, int axis, bool inverse, void* data, hipStream_t s);
 int set_variant(PlanCore& c, int axis, int variant);
+int set_fused(PlanCore& c, int on);
+size_t plan_device_bytes(const PlanCore& c);
 size_t local_bytes(const PlanCore& c);
 int device_cu_count();
 // Stage twiddle table of a schedule (layout of gfft_device.h), as bytes.

@@ -1,0 +1,264 @@
+// Fused z + y passes for x-planes that fit in LDS (small grids).
+//
+// One workgroup owns one x-plane: Ny rows of Nz reals (pitch 2K) = Ny x K
+// complex half spectra, contiguous in global memory.  R2C:
+//   0. the plane is copied into LDS (one coalesced read),
+//   1. z R2C of each row pair in LDS (two-for-one, the register engine's
+//      exchange runs inside the pair's own two rows),
+//   2. y C2C of each kz column in LDS (exchange through a column scratch),
+//   3. the plane is copied back (one coalesced write).
+// C2R runs 0, y C2C inverse, z C2R, and writes the Nz reals of every row.
+// The arithmetic (schedules, twiddles, split / Hermitian build, scale) is
+// that of the separate z and y passes, so the results agree to rounding (the
+// compiler may contract multiply-adds differently in the two kernels); the
+// plane makes one HBM round trip instead of two.
+#pragma once
+#include "gfft_kernels.h"
+
+namespace gfft {
+
+template <typename T>
+struct FlatLds {
+  cx<T>* b;
+  __device__ __forceinline__ cx<T>& at(int n) { return b[n]; }
+};
+
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW>
+struct PlanePass {
+  using C = cx<T>;
+  static constexpr int NZ = SCZ::N, NY = SCY::N, K = NZ / 2 + 1;
+  using EZF = RegEngine<T, false, SCZ, PZ, 2>;
+  using EZI = RegEngine<T, true, SCZ, PZ, 2>;
+  using EYF = RegEngine<T, false, SCY, PY, 0>;
+  using EYI = RegEngine<T, true, SCY, PY, 0>;
+  static constexpr int TPTZ = NZ / PZ, TPTY = NY / PY;
+  static constexpr int NT = CW * TPTY;
+  static constexpr int GZ = NT / TPTZ;  // row pairs in flight
+  static constexpr int NPAIRS = (NY + 1) / 2;
+  static constexpr int PLANE = NY * K;  // complex entries of one x-plane
+  static constexpr int SCRATCH = NY * CW;
+  static constexpr size_t LDS_BYTES = (size_t)(PLANE + SCRATCH) * sizeof(C);
+  static_assert(64 % TPTZ == 0, "the z exchange must stay inside one wavefront");
+  static_assert(NT % TPTZ == 0 && NT <= 1024, "thread count");
+  // the z exchange of an odd Ny's last (single-row) pair runs past the plane
+  // end by K - 2 (even Nz) entries, into the column scratch
+  static_assert(SCRATCH >= K, "column scratch");
+  static_assert(LDS_BYTES <= 160 * 1024, "plane does not fit in LDS");
+
+  // all of a thread's loads are issued before its LDS stores (U in flight)
+  __device__ __forceinline__ static void copy_in(C* plane, const C* src) {
+    constexpr int U = (PLANE + NT - 1) / NT < 16 ? (PLANE + NT - 1) / NT : 16;
+    for (int i0 = 0; i0 < PLANE; i0 += U * NT) {
+      C r[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * NT + (int)threadIdx.x;
+        if (i < PLANE) r[u] = ld_stream(src + i);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * NT + (int)threadIdx.x;
+        if (i < PLANE) plane[i] = r[u];
+      }
+    }
+  }
+
+  // z R2C of every row pair of the LDS plane, in place
+  __device__ __forceinline__ static void z_r2c(C* plane, const C* __restrict__ tw) {
+    const int gi = threadIdx.x / TPTZ, t = threadIdx.x % TPTZ;
+    constexpr int NI = (K + TPTZ - 1) / TPTZ;
+    for (int pair = gi; pair < NPAIRS; pair += GZ) {
+      const int ra = 2 * pair;
+      const bool vb = ra + 1 < NY;
+      T* A = reinterpret_cast<T*>(plane) + (size_t)ra * 2 * K;
+      const T* B = A + 2 * K;
+      C v[EZF::PV];
+#pragma unroll
+      for (int q = 0; q < EZF::NB0; ++q) {
+        if (!EZF::live0(t, q)) continue;
+#pragma unroll
+        for (int r = 0; r < EZF::R0; ++r) {
+          const int n = EZF::in_index(t, q, r);
+          v[q * EZF::R0 + r] = mk<T>(A[n], vb ? B[n] : (T)0);
+        }
+      }
+      FlatLds<T> L{reinterpret_cast<C*>(A)};
+      EZF::run(v, t, L, tw);
+      stage_barrier<2>();
+#pragma unroll
+      for (int q = 0; q < EZF::NBL; ++q) {
+        if (!EZF::liveL(t, q)) continue;
+#pragma unroll
+        for (int r = 0; r < EZF::RL; ++r) L.at(EZF::out_index(t, q, r)) = v[q * EZF::RL + r];
+      }
+      stage_barrier<2>();
+      const T h = (T)0.5;
+      C xa[NI], xb[NI];
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const int k = t + i * TPTZ;
+        if ((NI * TPTZ == K) || k < K) {
+          const C zk = L.at(k), zm = L.at(k == 0 ? 0 : NZ - k);
+          xa[i] = mk<T>((zk.x + zm.x) * h, (zk.y - zm.y) * h);
+          xb[i] = mk<T>((zk.y + zm.y) * h, (zm.x - zk.x) * h);
+        }
+      }
+      stage_barrier<2>();
+      C* Ac = reinterpret_cast<C*>(A);
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const int k = t + i * TPTZ;
+        if ((NI * TPTZ == K) || k < K) {
+          Ac[k] = xa[i];
+          if (vb) Ac[K + k] = xb[i];
+        }
+      }
+    }
+  }
+
+  // z C2R of every row pair of the LDS plane, in place (reals, pitch 2K)
+  __device__ __forceinline__ static void z_c2r(C* plane, const C* __restrict__ tw, T sc, bool scaled) {
+    using RC = RowsC2R<T, SCZ, PZ, 1>;
+    const int gi = threadIdx.x / TPTZ, t = threadIdx.x % TPTZ;
+    for (int pair = gi; pair < NPAIRS; pair += GZ) {
+      const int ra = 2 * pair;
+      const bool vb = ra + 1 < NY;
+      C* S = plane + (size_t)ra * K;  // half spectra of rows ra, ra + 1
+      C v[EZI::PV];
+#pragma unroll
+      for (int q = 0; q < EZI::NB0; ++q) {
+        if (!EZI::live0(t, q)) continue;
+#pragma unroll
+        for (int r = 0; r < EZI::R0; ++r) {
+          const int n = EZI::in_index(t, q, r);
+          if (vb) {
+            v[q * EZI::R0 + r] = RC::hermitian_z(S, n);
+          } else {  // a single row: its partner spectrum is zero
+            const bool lo = 2 * n <= NZ;
+            C xa = S[lo ? n : NZ - n];
+            if (n == 0 || 2 * n == NZ) xa.y = (T)0;
+            v[q * EZI::R0 + r] = lo ? xa : mk<T>(xa.x, -xa.y);
+          }
+        }
+      }
+      FlatLds<T> L{S};
+      EZI::run(v, t, L, tw);
+      stage_barrier<2>();  // every read of the pair's exchange region is done
+      T* A = reinterpret_cast<T*>(S);
+      T* B = A + 2 * K;
+#pragma unroll
+      for (int q = 0; q < EZI::NBL; ++q) {
+        if (!EZI::liveL(t, q)) continue;
+#pragma unroll
+        for (int r = 0; r < EZI::RL; ++r) {
+          const int n = EZI::out_index(t, q, r);
+          C z = v[q * EZI::RL + r];
+          if (scaled) z = mk<T>(z.x * sc, z.y * sc);
+          A[n] = z.x;
+          if (vb) B[n] = z.y;
+        }
+      }
+    }
+  }
+
+  // y C2C of every kz column of the LDS plane, in place
+  template <bool INV>
+  __device__ __forceinline__ static void y_pass(C* plane, C* scratch, const C* __restrict__ tw, T sc, bool scaled) {
+    using E = RegEngine<T, INV, SCY, PY, 0>;
+    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+    for (int k0 = 0; k0 < K; k0 += CW) {
+      const int k = k0 + c;
+      const bool valid = k < K;
+      C v[E::PV];
+#pragma unroll
+      for (int q = 0; q < E::NB0; ++q) {
+        if (!E::live0(t, q)) continue;
+#pragma unroll
+        for (int r = 0; r < E::R0; ++r)
+          v[q * E::R0 + r] = valid ? plane[E::in_index(t, q, r) * K + k] : mk<T>(0, 0);
+      }
+      ColLds<T, CW> L{scratch, c};
+      E::run(v, t, L, tw);
+      if (valid) {
+#pragma unroll
+        for (int q = 0; q < E::NBL; ++q) {
+          if (!E::liveL(t, q)) continue;
+#pragma unroll
+          for (int r = 0; r < E::RL; ++r) {
+            C z = v[q * E::RL + r];
+            if (scaled) z = mk<T>(z.x * sc, z.y * sc);
+            plane[E::out_index(t, q, r) * K + k] = z;
+          }
+        }
+      }
+    }
+  }
+};
+
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, PlanePass<T, SCZ, PZ, SCY, PY, CW>::NT))) void k_plane_r2c(
+    T* data, const cx<T>* twz, const cx<T>* twy, double yscale) {
+  using K_ = PlanePass<T, SCZ, PZ, SCY, PY, CW>;
+  __shared__ cx<T> smem[K_::PLANE + K_::SCRATCH];
+  cx<T>* g = reinterpret_cast<cx<T>*>(data) + (size_t)blockIdx.x * K_::PLANE;
+  K_::copy_in(smem, g);
+  __syncthreads();
+  K_::z_r2c(smem, twz);
+  __syncthreads();
+  K_::template y_pass<false>(smem, smem + K_::PLANE, twy, (T)yscale, yscale != 1.0);
+  __syncthreads();
+  for (int i = threadIdx.x; i < K_::PLANE; i += K_::NT) st_stream(g + i, smem[i]);
+}
+
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, PlanePass<T, SCZ, PZ, SCY, PY, CW>::NT))) void k_plane_c2r(
+    T* data, const cx<T>* twz, const cx<T>* twy, double zscale) {
+  using K_ = PlanePass<T, SCZ, PZ, SCY, PY, CW>;
+  __shared__ cx<T> smem[K_::PLANE + K_::SCRATCH];
+  cx<T>* g = reinterpret_cast<cx<T>*>(data) + (size_t)blockIdx.x * K_::PLANE;
+  K_::copy_in(smem, g);
+  __syncthreads();
+  K_::template y_pass<true>(smem, smem + K_::PLANE, twy, (T)1, false);
+  __syncthreads();
+  K_::z_c2r(smem, twz, (T)zscale, zscale != 1.0);
+  __syncthreads();
+  // the Nz reals of every row (the two pad reals of a row are unspecified
+  // after a C2R, as in the reference; here they keep the x pass's output)
+  T* gr = reinterpret_cast<T*>(g);
+  const T* sr = reinterpret_cast<const T*>(smem);
+  constexpr int NZ = K_::NZ, P2 = 2 * K_::K;
+  for (int i = threadIdx.x; i < K_::NY * NZ; i += K_::NT) {
+    const int y = i / NZ, n = i - y * NZ;
+    st_stream(gr + (size_t)y * P2 + n, sr[y * P2 + n]);
+  }
+}
+
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW, bool INV>
+hipError_t launch_plane(void* data, int64_t nplanes, double scale, const void* twz, const void* twy,
+                        hipStream_t s) {
+  using K_ = PlanePass<T, SCZ, PZ, SCY, PY, CW>;
+  if (nplanes <= 0) return hipSuccess;
+  if (INV)
+    hipLaunchKernelGGL((k_plane_c2r<T, SCZ, PZ, SCY, PY, CW>), dim3((uint32_t)nplanes), dim3(K_::NT), 0, s,
+                       static_cast<T*>(data), static_cast<const cx<T>*>(twz), static_cast<const cx<T>*>(twy),
+                       scale);
+  else
+    hipLaunchKernelGGL((k_plane_r2c<T, SCZ, PZ, SCY, PY, CW>), dim3((uint32_t)nplanes), dim3(K_::NT), 0, s,
+                       static_cast<T*>(data), static_cast<const cx<T>*>(twz), static_cast<const cx<T>*>(twy),
+                       scale);
+  return hipGetLastError();
+}
+
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW>
+PlaneEntry entry_plane() {
+  return PlaneEntry{prec_of<T>(),
+                    SCZ::N,
+                    SCY::N,
+                    make_sched<SCZ>(),
+                    make_sched<SCY>(),
+                    PlanePass<T, SCZ, PZ, SCY, PY, CW>::NT,
+                    reinterpret_cast<void*>(&launch_plane<T, SCZ, PZ, SCY, PY, CW, false>),
+                    reinterpret_cast<void*>(&launch_plane<T, SCZ, PZ, SCY, PY, CW, true>)};
+}
+
+}  // namespace gfft

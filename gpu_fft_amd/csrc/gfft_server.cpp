@@ -120,7 +120,7 @@ int get_plan(gfft_server_s* s, int64_t nx, int64_t ny, int64_t nz, int prec, con
     gfft::free_twiddles(*p);
     return st;
   }
-  s->plan_bytes += p->z.tw_bytes + p->y.tw_bytes + p->x.tw_bytes;
+  s->plan_bytes += gfft::plan_device_bytes(*p);
   *out = p.get();
   s->plans.emplace(k, std::move(p));
   return GFFT_OK;

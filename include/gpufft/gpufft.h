@@ -118,9 +118,11 @@ int gfft_exec_c2r_local(gfft_plan plan, const void* recvbuf, void* data, gfft_st
 /* Run one axis pass of the plan in place (axis 0 = z rows, 1 = y, 2 = x;
  * inverse 0 = the R2C direction, 1 = the C2R direction), for per-pass
  * measurement and for callers that interleave their own work between passes.
- * R2C = passes (z,0), (y,0), (x,0); C2R = (x,1), (y,1), (z,1).  A length-1
- * strided axis is a no-op.  No counterpart in the reference (its passes are
- * hipFFT plans, GPU_FFT_init.cu:86-91). */
+ * R2C = passes (z,0), (y,0), (x,0); C2R = (x,1), (y,1), (z,1).  Axis 3 is the
+ * fused z + y plane pass (R2C: z then y; C2R: y then z), valid only while the
+ * plan uses it (describe: "zy_fused"); then R2C = (3,0), (x,0) and C2R =
+ * (x,1), (3,1).  A length-1 strided axis is a no-op.  No counterpart in the
+ * reference (its passes are hipFFT plans, GPU_FFT_init.cu:86-91). */
 int gfft_exec_axis(gfft_plan plan, int axis, int inverse, void* data, gfft_stream_t stream);
 
 /* Autotuner: time every compiled variant of every axis pass (tile width,
@@ -148,6 +150,15 @@ int gfft_plan_set_chunking(gfft_plan plan, int64_t planes);
  * forward; C2R: x reversed, y forward, z reversed), which are still in the
  * memory-side cache.  Bitwise the same results either way. */
 int gfft_plan_set_walk(gfft_plan plan, int mode);
+
+/* Fused z + y kernels for grids whose x-planes fit in on-chip memory (small
+ * Ny x Nz, see gfft_plan_describe's "zy_plane_kernel"): each x-plane makes one
+ * HBM round trip for both axes instead of two.  1 (the default where such a
+ * kernel is compiled) = use them, 0 = separate passes; gfft_plan_tune keeps
+ * the faster.  Results agree to rounding either way.  Returns
+ * GFFT_ERR_INVALID_ARG for 1 where no fused kernel exists.  No reference
+ * counterpart (the reference always runs separate library passes). */
+int gfft_plan_set_fused(gfft_plan plan, int on);
 
 /* Output scale factors fused into the last pass of each direction (default
  * 1.0 and 1.0: unnormalised, the reference's semantics).  c2r_scale =
