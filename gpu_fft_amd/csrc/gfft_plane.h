@@ -17,10 +17,14 @@
 
 namespace gfft {
 
+// Exchange slots of a z transform inside its pair's own rows (N entries,
+// no room for padding): point n lives at n ^ ((n >> 3) & 7), a permutation
+// inside each block of 8 that spreads the stride-8 accesses of the first
+// stages over the LDS banks (N is a multiple of 8 for every plane size).
 template <typename T>
 struct FlatLds {
   cx<T>* b;
-  __device__ __forceinline__ cx<T>& at(int n) { return b[n]; }
+  __device__ __forceinline__ cx<T>& at(int n) { return b[n ^ ((n >> 3) & 7)]; }
 };
 
 template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW>
@@ -44,6 +48,7 @@ struct PlanePass {
   // end by K - 2 (even Nz) entries, into the column scratch
   static_assert(SCRATCH >= K, "column scratch");
   static_assert(LDS_BYTES <= 160 * 1024, "plane does not fit in LDS");
+  static_assert(NZ % 8 == 0, "FlatLds permutes inside blocks of 8 points");
 
   // all of a thread's loads are issued before its LDS stores (U in flight)
   __device__ __forceinline__ static void copy_in(C* plane, const C* src) {
