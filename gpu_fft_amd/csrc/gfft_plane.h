@@ -227,14 +227,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, PlanePass<T, SCZ, PZ, S
   __syncthreads();
   K_::z_c2r(smem, twz, (T)zscale, zscale != 1.0);
   __syncthreads();
-  // the Nz reals of every row (the two pad reals of a row are unspecified
-  // after a C2R, as in the reference; here they keep the x pass's output)
-  T* gr = reinterpret_cast<T*>(g);
-  const T* sr = reinterpret_cast<const T*>(smem);
-  constexpr int NZ = K_::NZ, P2 = 2 * K_::K;
-  for (int i = threadIdx.x; i < K_::NY * NZ; i += K_::NT) {
-    const int y = i / NZ, n = i - y * NZ;
-    st_stream(gr + (size_t)y * P2 + n, sr[y * P2 + n]);
+  // the Nz reals of every row, as Nz / 2 pairs (the two pad reals of a row
+  // are unspecified after a C2R, as in the reference; here they keep the x
+  // pass's output)
+  constexpr int H = K_::NZ / 2;
+  for (int i = threadIdx.x; i < K_::NY * H; i += K_::NT) {
+    const int y = i / H, m = i - y * H;
+    st_stream(g + y * K_::K + m, smem[y * K_::K + m]);
   }
 }
 
