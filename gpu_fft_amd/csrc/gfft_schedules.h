@@ -63,6 +63,13 @@ using Sc480 = Sched<8, 4, 5, 3>;
 using Sc486 = Sched<2, 3, 3, 3, 3, 3>;
 using Sc500 = Sched<4, 5, 5, 5>;
 
+// 7 x 2^k: 8s first, then 4s, radix 7 last.
+using Sc56 = Sched<8, 7>;
+using Sc112 = Sched<4, 4, 7>;
+using Sc224 = Sched<8, 4, 7>;
+using Sc448 = Sched<8, 8, 7>;
+using Sc896 = Sched<8, 4, 4, 7>;
+
 // Default variant of a size: z R2C, z C2R, strided forward, strided inverse.
 // The first variant registered for a (precision, kind, N) is the default;
 // later ones are tuning alternatives (gfft_plan_tune / gfft_plan_set_variant).

@@ -54,7 +54,8 @@ def _run_axis(shape, prec, axis, inverse, seed=1, variant=0):
 
 
 SHAPES = [(16, 768, 24), (8, 768, 16), (768, 8, 16), (4, 4, 768), (32, 512, 24), (512, 16, 16), (8, 8, 512),
-          (24, 24, 24), (16, 384, 8), (8, 1024, 8), (1024, 8, 8), (4, 4, 1024), (4, 2048, 4), (4, 4, 2048)]
+          (24, 24, 24), (16, 384, 8), (8, 1024, 8), (1024, 8, 8), (4, 4, 1024), (4, 2048, 4), (4, 4, 2048),
+          (4, 448, 16), (448, 4, 16), (4, 4, 448), (4, 896, 8), (896, 4, 8), (4, 4, 896)]
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])

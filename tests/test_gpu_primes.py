@@ -14,7 +14,8 @@ from oracle import gfft_oracle as O
 from test_gpu_sizes import TOL, axis_error
 
 pytestmark = pytest.mark.gpu
-EDGES = [7, 11, 13, 14, 21, 49, 77, 91, 143, 169, 343, 448, 1001]
+EDGES = [7, 11, 13, 14, 21, 49, 56, 77, 91, 112, 143, 169, 224, 343, 448, 896, 1001]
+COMPILED = {56, 112, 224, 448, 896}  # 7 x 2^k: register-engine variants
 
 
 def say(*a):
@@ -31,6 +32,7 @@ def test_prime_edge_every_pass(n, prec):
         for inverse in (False, True):
             err, eng = axis_error(torch, shape, prec, axis, inverse, seed=3 * n + axis)
             assert err <= TOL[prec], (n, prec, shape, axis, inverse, eng, err)
+            assert (eng == "register") == (n in COMPILED), (n, prec, axis, eng)
 
 
 @pytest.mark.parametrize("shape", [(7, 11, 13), (13, 7, 22), (11, 13, 14)])
