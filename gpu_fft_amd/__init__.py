@@ -19,7 +19,8 @@ import os
 
 __all__ = [
     "GfftError", "Plan", "lib", "lib_path", "query_plan", "kdim", "alloc_inplace",
-    "real_view", "complex_view", "F32", "F64",
+    "real_view", "complex_view", "F32", "F64", "tune_cache_text", "load_tune_cache_text", "clear_tune_cache",
+    "save_tune_cache", "load_tune_cache",
 ]
 
 F32 = 0
@@ -76,6 +77,12 @@ def lib() -> ctypes.CDLL:
     L.gfft_plan_set_variant.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.gfft_plan_set_fused.restype = ctypes.c_int
     L.gfft_plan_set_fused.argtypes = [vp, ctypes.c_int]
+    L.gfft_tune_cache_export.restype = ctypes.c_int
+    L.gfft_tune_cache_export.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+    L.gfft_tune_cache_import.restype = ctypes.c_int
+    L.gfft_tune_cache_import.argtypes = [ctypes.c_char_p]
+    L.gfft_tune_cache_clear.restype = ctypes.c_int
+    L.gfft_tune_cache_clear.argtypes = []
     L.gfft_plan_set_chunking.restype = ctypes.c_int
     L.gfft_plan_set_chunking.argtypes = [vp, ctypes.c_int64]
     L.gfft_plan_set_walk.restype = ctypes.c_int
@@ -126,6 +133,35 @@ def query_plan(nx, ny, nz, procs=1, rank=0, precision="f32") -> dict:
     buf = ctypes.create_string_buffer(8192)
     _check(lib().gfft_query_plan(nx, ny, nz, procs, rank, _prec(precision), buf, len(buf)), "gfft_query_plan")
     return json.loads(buf.value.decode())
+
+
+def tune_cache_text() -> str:
+    """The process-wide autotuning cache (what gfft_plan_tune has chosen so
+    far), one "<device>|<geometry> <kernel>" line per tuned pass."""
+    need = ctypes.c_size_t(0)
+    _check(lib().gfft_tune_cache_export(None, 0, ctypes.byref(need)), "gfft_tune_cache_export")
+    buf = ctypes.create_string_buffer(max(1, need.value))
+    _check(lib().gfft_tune_cache_export(buf, len(buf), None), "gfft_tune_cache_export")
+    return buf.value.decode()
+
+
+def load_tune_cache_text(text: str) -> None:
+    """Add the entries of an exported cache (later plans skip their timing)."""
+    _check(lib().gfft_tune_cache_import(text.encode()), "gfft_tune_cache_import")
+
+
+def clear_tune_cache() -> None:
+    _check(lib().gfft_tune_cache_clear(), "gfft_tune_cache_clear")
+
+
+def save_tune_cache(path) -> None:
+    with open(path, "w") as fh:
+        fh.write("# gpu_fft_amd tuning cache\n" + tune_cache_text())
+
+
+def load_tune_cache(path) -> None:
+    with open(path) as fh:
+        load_tune_cache_text(fh.read())
 
 
 def _stream_handle(stream):

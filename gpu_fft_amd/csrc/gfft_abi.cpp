@@ -149,6 +149,23 @@ int gfft_plan_set_chunking(gfft_plan plan, int64_t planes) {
   return GFFT_OK;
 }
 
+int gfft_tune_cache_export(char* buf, size_t len, size_t* needed) {
+  const std::string t = gfft::tune_cache_text();
+  if (needed) *needed = t.size() + 1;
+  if (!buf) return needed ? GFFT_OK : GFFT_ERR_INVALID_ARG;
+  return copy_text(t, buf, len);
+}
+
+int gfft_tune_cache_import(const char* text) {
+  if (!text) return GFFT_ERR_INVALID_ARG;
+  return gfft::tune_cache_load(text);
+}
+
+int gfft_tune_cache_clear(void) {
+  gfft::tune_cache_clear();
+  return GFFT_OK;
+}
+
 int gfft_plan_set_fused(gfft_plan plan, int on) {
   if (!plan) return GFFT_ERR_INVALID_ARG;
   return gfft::set_fused(plan->core, on);

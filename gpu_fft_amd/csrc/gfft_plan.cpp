@@ -528,8 +528,29 @@ int set_fused(PlanCore& c, int on) {
 }
 
 namespace {
+// Tuning results, process-wide: key = device tag | geometry, value = the
+// chosen candidate's description (not its index, so an exported cache stays
+// meaningful for a library with a different variant list).
 std::mutex g_tune_mu;
-std::map<std::string, int> g_tune_cache;  // geometry key -> variant index
+std::map<std::string, std::string> g_tune_cache;
+
+// "<arch>/<CUs>" of the current device (cached per device ordinal)
+std::string device_tag() {
+  static std::mutex mu;
+  static std::map<int, std::string> tags;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return "unknown";
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = tags.find(dev);
+  if (it != tags.end()) return it->second;
+  hipDeviceProp_t p{};
+  std::string t = "unknown";
+  if (hipGetDeviceProperties(&p, dev) == hipSuccess) t = std::string(p.gcnArchName) + "/" + std::to_string(p.multiProcessorCount);
+  for (char& ch : t)
+    if (ch == ' ' || ch == '|') ch = '_';
+  tags[dev] = t;
+  return t;
+}
 
 std::string tune_key(const PlanCore& c, int axis) {
   char b[256];
@@ -541,9 +562,54 @@ std::string tune_key(const PlanCore& c, int axis) {
     std::snprintf(b, sizeof b, "p%d:cols:%d:%u:%u:%lld:%lld", c.prec, axis == 1 ? c.y.N : c.x.N, g.ncols,
                   g.in.inner, (long long)g.in.outer, (long long)g.in.rlo);
   }
+  return device_tag() + "|" + b;
+}
+
+// description of candidate v of an axis: kernel shape and radix order
+std::string cand_desc(const AxisPlan& a, int v) {
+  const KernelEntry* e = a.cands[v].first;
+  if (!e) return "generic";
+  std::string r;
+  for (int i = 0; i < e->sched.S; ++i) r += (i ? "." : "") + std::to_string(e->sched.R[i]);
+  char b[160];
+  std::snprintf(b, sizeof b, "P%d:T%d:W%d:K%d:O%d:A%d:R%s", e->P, e->tile, e->threads, e->persist, e->order,
+                e->readdr, r.c_str());
   return b;
 }
 }  // namespace
+
+std::string tune_cache_text() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  std::string out;
+  for (const auto& kv : g_tune_cache) out += kv.first + " " + kv.second + "\n";
+  return out;
+}
+
+int tune_cache_load(const char* text) {
+  std::map<std::string, std::string> add;
+  const char* p = text;
+  while (*p) {
+    const char* e = p;
+    while (*e && *e != '\n') ++e;
+    std::string line(p, e);
+    p = *e ? e + 1 : e;
+    while (!line.empty() && (line.back() == '\r' || line.back() == ' ')) line.pop_back();
+    if (line.empty() || line[0] == '#') continue;
+    const size_t sp = line.find(' ');
+    if (sp == std::string::npos || sp == 0 || sp + 1 >= line.size() ||
+        line.find(' ', sp + 1) != std::string::npos || line.find('|') == std::string::npos)
+      return GFFT_ERR_INVALID_ARG;
+    add[line.substr(0, sp)] = line.substr(sp + 1);
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (auto& kv : add) g_tune_cache[kv.first] = kv.second;
+  return GFFT_OK;
+}
+
+void tune_cache_clear() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache.clear();
+}
 
 // Time every candidate of every axis (forward + inverse pass on `scratch`,
 // whose contents are destroyed) and keep the fastest.  Results are cached per
@@ -564,8 +630,12 @@ int tune(PlanCore& c, void* scratch, hipStream_t s, int reps) {
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tune_cache.find(key);
-      if (it != g_tune_cache.end() && it->second < (int)a->cands.size()) {
-        set_variant(c, axis, it->second);
+      int hit = -1;
+      if (it != g_tune_cache.end())
+        for (int v = 0; v < (int)a->cands.size() && hit < 0; ++v)
+          if (cand_desc(*a, v) == it->second) hit = v;
+      if (hit >= 0) {
+        set_variant(c, axis, hit);
         continue;
       }
     }
@@ -598,18 +668,20 @@ int tune(PlanCore& c, void* scratch, hipStream_t s, int reps) {
     set_variant(c, axis, best);
     a->tuned_ms = best_ms;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[key] = best;
+    g_tune_cache[key] = cand_desc(*a, best);
   }
   // fused z + y plane kernels against the (tuned) separate passes
   if (st == GFFT_OK && c.plane && c.d_twz_plane) {
-    char kb[128];
-    std::snprintf(kb, sizeof kb, "p%d:plane:%lld:%lld:%lld", c.prec, (long long)c.nx, (long long)c.ny,
+    char kb0[128];
+    std::snprintf(kb0, sizeof kb0, "p%d:plane:%lld:%lld:%lld", c.prec, (long long)c.nx, (long long)c.ny,
                   (long long)c.nz);
+    const std::string kb = device_tag() + "|" + kb0;
     int cached = -1;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tune_cache.find(kb);
-      if (it != g_tune_cache.end()) cached = it->second;
+      if (it != g_tune_cache.end() && (it->second == "fused:0" || it->second == "fused:1"))
+        cached = it->second == "fused:1" ? 1 : 0;
     }
     if (cached >= 0) {
       c.fused = cached;
@@ -640,7 +712,7 @@ int tune(PlanCore& c, void* scratch, hipStream_t s, int reps) {
         c.fused_ms = ms[1];
         c.fused = ms[1] < ms[0] ? 1 : 0;
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune_cache[kb] = c.fused;
+        g_tune_cache[kb] = c.fused ? "fused:1" : "fused:0";
       }
     }
   }

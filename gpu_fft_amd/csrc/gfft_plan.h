@@ -58,6 +58,10 @@ int exec_c2r(const PlanCore& c, void* data, hipStream_t s);
 int exec_axis(const PlanCore& c, int axis, bool inverse, void* data, hipStream_t s);
 int set_variant(PlanCore& c, int axis, int variant);
 int set_fused(PlanCore& c, int on);
+// process-wide tuning cache as text lines "<device>|<geometry> <choice>"
+std::string tune_cache_text();
+int tune_cache_load(const char* text);
+void tune_cache_clear();
 size_t plan_device_bytes(const PlanCore& c);
 size_t local_bytes(const PlanCore& c);
 int device_cu_count();

@@ -151,6 +151,19 @@ int gfft_plan_set_chunking(gfft_plan plan, int64_t planes);
  * memory-side cache.  Bitwise the same results either way. */
 int gfft_plan_set_walk(gfft_plan plan, int mode);
 
+/* Persistent autotuning: the process-wide cache that gfft_plan_tune fills
+ * and consults, as text, one line per tuned geometry:
+ *   "<gfx arch>/<CUs>|<pass geometry> <chosen kernel description>"
+ * Export (buf = NULL queries the size into *needed, NUL included; a short
+ * buffer gives GFFT_ERR_BUFFER_TOO_SMALL), import (adds or replaces entries;
+ * '#' lines are comments; a malformed line rejects the whole text with
+ * GFFT_ERR_INVALID_ARG), clear.  Entries of another device type or of a
+ * kernel the library does not have are ignored at lookup (the plan is then
+ * tuned again).  No counterpart in the reference (it has no tuning). */
+int gfft_tune_cache_export(char* buf, size_t len, size_t* needed);
+int gfft_tune_cache_import(const char* text);
+int gfft_tune_cache_clear(void);
+
 /* Fused z + y kernels for grids whose x-planes fit in on-chip memory (small
  * Ny x Nz, see gfft_plan_describe's "zy_plane_kernel"): each x-plane makes one
  * HBM round trip for both axes instead of two.  1 (the default where such a

@@ -132,3 +132,27 @@ def test_13_smooth_sizes_plan(n):
     for ax in ("z", "y", "x"):
         r = d[ax]["radices"]
         assert math.prod(r) == n and set(r) <= {2, 3, 4, 5, 7, 8, 11, 13}
+
+
+def test_tune_cache_text_round_trip_and_validation():
+    """The tuning cache imports, exports and rejects malformed text without a
+    GPU (entries are only consulted by gfft_plan_tune)."""
+    G.clear_tune_cache()
+    assert G.tune_cache_text() == ""
+    text = ("# comment\n"
+            "gfx950/256|p0:rows:512:262144 P8:T4:W256:K2:O1:A0:R8.8.8\n"
+            "gfx950/256|p0:plane:128:128:128 fused:1\n")
+    G.load_tune_cache_text(text)
+    got = G.tune_cache_text().splitlines()
+    assert sorted(got) == sorted(l for l in text.splitlines() if not l.startswith("#"))
+    for bad in ("no-separator-or-value\n", "gfx950/256|key a b\n", "nodevicekey value\n"):
+        with pytest.raises(G.GfftError) as e:
+            G.load_tune_cache_text(bad)
+        assert e.value.status == 1
+    assert len(G.tune_cache_text().splitlines()) == 2  # a rejected text adds nothing
+    need = ctypes.c_size_t(0)
+    assert G.lib().gfft_tune_cache_export(None, 0, ctypes.byref(need)) == 0
+    small = ctypes.create_string_buffer(4)
+    assert G.lib().gfft_tune_cache_export(small, 4, None) == 9
+    G.clear_tune_cache()
+    assert G.tune_cache_text() == ""
