@@ -91,7 +91,7 @@ def measured_traffic(pass_name: str, n: int, prec: str):
 def slab_share(args, nx, ny, nz, procs):
     """One rank's share of a slab decomposition: the local R2C half (z R2C +
     y C2C with the slab-transpose pack fused into the y stores), on the
-    (Nx/p, Ny, 2K) block.  Algorithmic bytes: SURVEY's unfused model 6 S_local
+    (Nx/p, Ny, 2K) block.  Algorithmic bytes: the unfused model's 6 S_local
     (z, y, pack each read + write), the fused data movement is 4 S_local."""
     import torch
     import gpu_fft_amd as G
