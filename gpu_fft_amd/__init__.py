@@ -395,7 +395,11 @@ class Batch:
         _check(lib().gfft_batch_tune(self.handle, int(mode), int(reps), _stream_handle(stream)), "gfft_batch_tune")
         return self.variants()
 
+    PLAIN = 3  # variant: the class's transforms through single-transform plans
+
     def set_variant(self, edge: int, variant: int) -> None:
+        """Variant of a size class: 0-2 = level kernels (radix order, points
+        per thread), 3 = per-transform plans."""
         _check(lib().gfft_batch_set_variant(self.handle, int(edge), int(variant)), "gfft_batch_set_variant")
 
     def variants(self) -> dict:

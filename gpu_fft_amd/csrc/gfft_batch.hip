@@ -13,11 +13,18 @@
 // always drains.
 //
 // Each class has kVariants compiled variants (radix order, and points per
-// thread, which sets the pencils per 256-thread tile); gfft_batch_tune times
-// them on the batch's own data and keeps the fastest per class.  Every tile
-// runs the same register-engine tile code as the single-transform kernels.
+// thread, which sets the pencils per 256-thread tile) plus one more choice,
+// kPlainVariant: the class's transforms run one after another through
+// single-transform plans (persistent pass kernels, fused z+y plane kernels),
+// which beats level launches for a few large transforms.  gfft_batch_tune
+// times every choice on the batch's own data and keeps the fastest per
+// class.  Every tile runs the same register-engine tile code as the
+// single-transform kernels.
 #include <algorithm>
+#include <map>
+#include <memory>
 #include <new>
+#include <tuple>
 #include <vector>
 
 #include "gfft_kernels.h"
@@ -30,6 +37,7 @@ namespace {
 constexpr int kBT = 256;  // threads per workgroup, every tile kind
 constexpr int kClasses = 8;
 constexpr int kVariants = 3;
+constexpr int kPlainVariant = kVariants;  // per-transform plans
 constexpr int kMaxLevels = 6;
 constexpr int kLanes = 4;  // internal streams of the eager path
 
@@ -260,6 +268,9 @@ struct gfft_batch_s {
   hipEvent_t ev_join[gfft::kLanes] = {};
   hipGraph_t graph[3] = {};
   hipGraphExec_t gexec[3] = {};
+  // single-transform plans (one per distinct geometry) of the plain variant
+  std::map<std::tuple<int64_t, int64_t, int64_t>, std::unique_ptr<gfft::PlanCore>> plans;
+  std::vector<const gfft::PlanCore*> xplan;  // per sorted position
 };
 
 namespace {
@@ -284,8 +295,16 @@ void drop_graphs(gfft_batch_s* b) {
   }
 }
 
+bool any_plain(const gfft_batch_s* b) {
+  for (int c = 0; c < gfft::kClasses; ++c)
+    if (b->n[c] && b->var[c] == gfft::kPlainVariant) return true;
+  return false;
+}
+
 void release(gfft_batch_s* b) {
   drop_graphs(b);
+  for (auto& kv : b->plans) gfft::free_twiddles(*kv.second);
+  b->plans.clear();
   for (int i = 0; i < gfft::kLanes; ++i) {
     if (b->ev_join[i]) (void)hipEventDestroy(b->ev_join[i]);
     if (b->ws[i]) (void)hipStreamDestroy(b->ws[i]);
@@ -315,8 +334,25 @@ uint32_t level_grid(const gfft_batch_s* b, int c, uint32_t total) {
   return std::min<uint32_t>(total, (uint32_t)(b->cus * b->occ[c][b->var[c]]));
 }
 
+// The plain variant: the class's transforms one after another through their
+// single-transform plans, each with its own output scales.
+int launch_plain(const gfft_batch_s* b, int c, int mode, hipStream_t s) {
+  for (int k = 0; k < b->n[c]; ++k) {
+    const BXform& x = b->hx[b->off[c] + k];
+    gfft::PlanCore pc = *b->xplan[b->off[c] + k];  // shares the plan's device tables
+    pc.r2c_scale = x.scale_r2c;
+    pc.c2r_scale = x.scale_c2r;
+    int st = GFFT_OK;
+    if (mode != 1) st = gfft::exec_r2c(pc, x.data, s);
+    if (st == GFFT_OK && mode != 0) st = gfft::exec_c2r(pc, x.data, s);
+    if (st != GFFT_OK) return st;
+  }
+  return GFFT_OK;
+}
+
 // The level chain of one class on one stream.
 int launch_chain(const gfft_batch_s* b, int c, int mode, hipStream_t s) {
+  if (b->var[c] == gfft::kPlainVariant) return launch_plain(b, c, mode, s);
   for (int l = 0; l < kModeLevels[mode]; ++l) {
     const gfft::LevelArgs a = level_args(b, c, mode, l);
     if (a.total == 0) continue;
@@ -347,13 +383,35 @@ int launch_eager(gfft_batch_s* b, int mode, hipStream_t s) {
   return GFFT_OK;
 }
 
-// Graph path: one kernel node per (class, level); each class is a chain.
+// Graph path: one kernel node per (class, level); each class is a chain.  A
+// class on the plain variant is one child-graph node captured from its plan
+// launches (its data pointers and scales are then part of the graph, so
+// set_data / set_scales drop the graphs).
 int build_graph(gfft_batch_s* b, int mode, hipGraph_t* out) {
   using namespace gfft;
   hipGraph_t g = nullptr;
   if (hipGraphCreate(&g, 0) != hipSuccess) return GFFT_ERR_HIP;
   for (int c = 0; c < kClasses; ++c) {
     if (b->n[c] == 0) continue;
+    if (b->var[c] == kPlainVariant) {
+      hipStream_t cs = b->ws[b->lane[c]];
+      hipGraph_t child = nullptr;
+      if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return GFFT_ERR_HIP;
+      }
+      const int st = launch_plain(b, c, mode, cs);
+      const hipError_t ce = hipStreamEndCapture(cs, &child);
+      hipGraphNode_t node = nullptr;
+      if (st != GFFT_OK || ce != hipSuccess || !child ||
+          hipGraphAddChildGraphNode(&node, g, nullptr, 0, child) != hipSuccess) {
+        if (child) (void)hipGraphDestroy(child);
+        (void)hipGraphDestroy(g);
+        return st != GFFT_OK ? st : GFFT_ERR_HIP;
+      }
+      (void)hipGraphDestroy(child);
+      continue;
+    }
     hipGraphNode_t prev = nullptr;
     for (int l = 0; l < kModeLevels[mode]; ++l) {
       LevelArgs a = level_args(b, c, mode, l);
@@ -511,6 +569,28 @@ int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec)
       st = GFFT_ERR_HIP;
   }
   if (st == GFFT_OK && hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess) st = GFFT_ERR_HIP;
+  // single-transform plans of the plain variant, one per distinct geometry
+  b->xplan.assign(count, nullptr);
+  for (int j = 0; j < count && st == GFFT_OK; ++j) {
+    const int i = order[j];
+    const auto key = std::make_tuple(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]);
+    auto it = b->plans.find(key);
+    if (it == b->plans.end()) {
+      std::unique_ptr<PlanCore> pc(new (std::nothrow) PlanCore);
+      if (!pc) {
+        st = GFFT_ERR_OUT_OF_MEMORY;
+        break;
+      }
+      st = plan_host(*pc, dims[3 * i], dims[3 * i + 1], dims[3 * i + 2], 1, 0, F32);
+      if (st == GFFT_OK) st = upload_twiddles(*pc);
+      if (st != GFFT_OK) {
+        free_twiddles(*pc);
+        break;
+      }
+      it = b->plans.emplace(key, std::move(pc)).first;
+    }
+    b->xplan[j] = it->second.get();
+  }
   if (st == GFFT_OK &&
       hipMemcpy(b->d_prefix, prefix.data(), sizeof(uint32_t) * prefix.size(), hipMemcpyHostToDevice) != hipSuccess)
     st = GFFT_ERR_HIP;
@@ -529,7 +609,8 @@ int gfft_batch_set_data(gfft_batch b, void* const* ptrs) {
   for (int i = 0; i < b->count; ++i)
     if (!ptrs[i]) return GFFT_ERR_INVALID_ARG;
   for (int i = 0; i < b->count; ++i) b->hx[b->pos[i]].data = static_cast<float*>(ptrs[i]);
-  const int st = upload_xf(b);
+  const int st = upload_xf(b);  // synchronises the device first
+  if (any_plain(b)) drop_graphs(b);
   if (st == GFFT_OK) b->data_set = true;
   return st;
 }
@@ -540,7 +621,9 @@ int gfft_batch_set_scales(gfft_batch b, const double* r2c_scale, const double* c
     b->hx[b->pos[i]].scale_r2c = r2c_scale ? (float)r2c_scale[i] : 1.0f;
     b->hx[b->pos[i]].scale_c2r = c2r_scale ? (float)c2r_scale[i] : 1.0f;
   }
-  return upload_xf(b);
+  const int st = upload_xf(b);
+  if (any_plain(b)) drop_graphs(b);
+  return st;
 }
 
 int gfft_batch_exec(gfft_batch b, int mode, gfft_stream_t stream) {
@@ -571,7 +654,7 @@ int gfft_batch_exec_graph(gfft_batch b, int mode, gfft_stream_t stream) {
 }
 
 int gfft_batch_set_variant(gfft_batch b, int64_t edge, int variant) {
-  if (!b || variant < 0 || variant >= gfft::kVariants) return GFFT_ERR_INVALID_ARG;
+  if (!b || variant < 0 || variant > gfft::kPlainVariant) return GFFT_ERR_INVALID_ARG;
   const int c = gfft::class_of(edge);
   if (c < 0) return GFFT_ERR_INVALID_ARG;
   if (hipDeviceSynchronize() != hipSuccess) return GFFT_ERR_HIP;
@@ -604,7 +687,7 @@ int gfft_batch_tune(gfft_batch b, int mode, int reps, gfft_stream_t stream) {
     if (b->n[c] == 0) continue;
     int best = 0;
     float best_ms = 1e30f;
-    for (int v = 0; v < gfft::kVariants && st == GFFT_OK; ++v) {
+    for (int v = 0; v <= gfft::kPlainVariant && st == GFFT_OK; ++v) {
       b->var[c] = v;
       if ((st = launch_chain(b, c, mode, s)) != GFFT_OK) break;  // warm-up
       if (hipEventRecord(e0, s) != hipSuccess) st = GFFT_ERR_HIP;
@@ -636,7 +719,7 @@ int gfft_batch_tiles(gfft_batch b, int mode, uint32_t* tiles, uint32_t* launches
   if (!b || !tiles || mode < 0 || mode > 2) return GFFT_ERR_INVALID_ARG;
   uint32_t t = 0, n = 0;
   for (int c = 0; c < gfft::kClasses; ++c)
-    for (int l = 0; l < kModeLevels[mode]; ++l) {
+    for (int l = 0; l < kModeLevels[mode] && b->var[c] != gfft::kPlainVariant; ++l) {
       const uint32_t k = b->level_tiles[c][b->var[c]][mode][l];
       t += k;
       n += k ? 1u : 0u;

@@ -161,10 +161,11 @@ def test_batch_rejects_unsupported_sizes():
         G.Batch([(32, 64, 32)])
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_every_variant_matches_oracle(variant):
     """Each compiled variant (radix order, pencils per tile) of every size
-    class, forced for all classes, against the FP64 oracle."""
+    class and the per-transform-plan variant (3), forced for all classes,
+    against the FP64 oracle."""
     import torch
     dims = [(n, n, n) for n in SIZES if n <= 192] + [(1, 256, 256), (256, 1, 256)]
     bufs, host = inputs(torch, dims, 300 + variant)
@@ -189,7 +190,7 @@ def test_tuned_batch_matches_oracle():
     with G.Batch(dims) as b:
         b.set_data(scratch)
         chosen = b.tune(G.ROUND_TRIP, reps=2)
-        assert set(chosen) == {d[2] for d in dims} and all(0 <= v < 3 for v in chosen.values())
+        assert set(chosen) == {d[2] for d in dims} and all(0 <= v <= 3 for v in chosen.values())
         bufs, host = inputs(torch, dims, 600)
         b.set_data(bufs)
         b.exec_graph(G.R2C)
@@ -198,3 +199,29 @@ def test_tuned_batch_matches_oracle():
             got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
             err = O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), nz))
             assert err <= 1e-6, ((nx, ny, nz), err)
+
+
+def test_plain_variant_in_graph_follows_new_data_and_scales():
+    """Classes on the per-transform-plan variant are captured into the graph
+    with their data pointers and scales: set_data / set_scales rebuild it,
+    so replays on new arrays and scales are right (vs the eager result)."""
+    import torch
+    dims = mix(10, seed=8, max_edge=128) + [(160, 160, 160)]
+    with G.Batch(dims) as b:
+        for n in SIZES:
+            b.set_variant(n, G.Batch.PLAIN)
+        for rep, sc in enumerate((1.0, 0.5)):
+            bufs, host = inputs(torch, dims, 700 + rep)
+            eager = [x.clone() for x in bufs]
+            scales = [sc / (nx * ny * nz) for nx, ny, nz in dims]
+            b.set_scales(c2r=scales)
+            b.set_data(eager)
+            b.exec(G.ROUND_TRIP)
+            torch.cuda.synchronize()
+            b.set_data(bufs)
+            b.exec_graph(G.ROUND_TRIP)
+            torch.cuda.synchronize()
+            for w, e, fp, (nx, ny, nz) in zip(bufs, eager, host, dims):
+                assert torch.equal(w, e)
+                err = O.rel_l2(w.cpu().numpy()[..., :nz], sc * fp[..., :nz])
+                assert err <= 1e-6, ((nx, ny, nz), rep, err)
