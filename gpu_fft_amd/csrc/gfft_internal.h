@@ -111,6 +111,8 @@ int registry_smooth_f64_a(const KernelEntry** out);
 int registry_smooth_f64_b(const KernelEntry** out);
 int registry_seven_f32(const KernelEntry** out);
 int registry_seven_f64(const KernelEntry** out);
+int registry_prime_f32(const KernelEntry** out);
+int registry_prime_f64(const KernelEntry** out);
 
 // Run-time-schedule (generic) launchers.
 struct RtSchedHost {

@@ -77,9 +77,9 @@ const std::vector<KernelEntry>& all_entries(int prec) {
     return v;
   };
   static const std::vector<KernelEntry> f32 =
-      gather({registry_f32, registry_smooth_f32_a, registry_smooth_f32_b, registry_seven_f32});
+      gather({registry_f32, registry_smooth_f32_a, registry_smooth_f32_b, registry_seven_f32, registry_prime_f32});
   static const std::vector<KernelEntry> f64 =
-      gather({registry_f64, registry_smooth_f64_a, registry_smooth_f64_b, registry_seven_f64});
+      gather({registry_f64, registry_smooth_f64_a, registry_smooth_f64_b, registry_seven_f64, registry_prime_f64});
   return prec == F32 ? f32 : f64;
 }
 

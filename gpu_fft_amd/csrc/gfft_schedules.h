@@ -70,6 +70,20 @@ using Sc224 = Sched<8, 4, 7>;
 using Sc448 = Sched<8, 8, 7>;
 using Sc896 = Sched<8, 4, 4, 7>;
 
+// 3 x 7 x 2^k, 11 x 2^k, 13 x 2^k: 8s and 4s first, the odd primes last.
+using Sc84 = Sched<4, 3, 7>;
+using Sc168 = Sched<8, 3, 7>;
+using Sc336 = Sched<4, 4, 3, 7>;
+using Sc672 = Sched<8, 4, 3, 7>;
+using Sc88 = Sched<8, 11>;
+using Sc176 = Sched<4, 4, 11>;
+using Sc352 = Sched<8, 4, 11>;
+using Sc704 = Sched<8, 8, 11>;
+using Sc104 = Sched<8, 13>;
+using Sc208 = Sched<4, 4, 13>;
+using Sc416 = Sched<8, 4, 13>;
+using Sc832 = Sched<8, 8, 13>;
+
 // Default variant of a size: z R2C, z C2R, strided forward, strided inverse.
 // The first variant registered for a (precision, kind, N) is the default;
 // later ones are tuning alternatives (gfft_plan_tune / gfft_plan_set_variant).

@@ -14,8 +14,10 @@ from oracle import gfft_oracle as O
 from test_gpu_sizes import TOL, axis_error
 
 pytestmark = pytest.mark.gpu
-EDGES = [7, 11, 13, 14, 21, 49, 56, 77, 91, 112, 143, 169, 224, 343, 448, 896, 1001]
-COMPILED = {56, 112, 224, 448, 896}  # 7 x 2^k: register-engine variants
+EDGES = [7, 11, 13, 14, 21, 49, 56, 77, 84, 88, 91, 104, 112, 143, 168, 169, 176, 208, 224, 336, 343, 352, 416,
+         448, 672, 704, 832, 896, 1001]
+# register-engine variants: 7 x 2^k, 3 x 7 x 2^k, 11 x 2^k, 13 x 2^k
+COMPILED = {56, 112, 224, 448, 896, 84, 168, 336, 672, 88, 176, 352, 704, 104, 208, 416, 832}
 
 
 def say(*a):
