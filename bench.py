@@ -575,7 +575,7 @@ def main():
                                 "compulsory_frac": round(4 * S / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
         "per_pass": per_pass,
         "variants": {ax: {k: desc[ax][k] for k in ("engine", "radices", "points_per_thread", "tile", "threads",
-                                                   "persist", "order", "readdr", "variant")} for ax in ("z", "y", "x")},
+                                                   "persist", "order", "readdr", "straight", "variant")} for ax in ("z", "y", "x")},
         "zy_fused": int(desc.get("zy_fused", 0)),
         "wall_ms_per_step": round(wall / args.steps * 1e3, 5),
         "output_rms": round(rms, 5),

@@ -87,6 +87,7 @@ struct KernelEntry {
   int order;        // tile order: persistent 0 contiguous ranges, 1 round robin; one tile per
                     // workgroup 0 XCD-aware ranges, 2 linear
   int readdr = 0;   // persistent columns: 1 = element offsets recomputed per tile (not hoisted)
+  int straight = 0; // persistent columns: 1 = straight-line loop for full tiles, single-stride maps
 };
 
 // Fused z + y kernel pair for x-planes that fit in LDS (gfft_plane.h).

@@ -275,6 +275,48 @@ struct ColsPass {
     }
   }
 
+  // Straight-line forms for tiles whose CW columns all exist: no per-lane
+  // guard and the row map's form decided once per tile, so no path skips a
+  // load or a store.  The compiler's wait counting is then exact across a
+  // persistent tile loop (it waits for the prefetched loads only, not for
+  // the previous tile's stores).
+  template <bool TWO>
+  __device__ __forceinline__ static int64_t row_off(const ColMap& m, int n) {
+    return TWO ? (int64_t)(n % m.nsub) * m.rlo + (int64_t)(n / m.nsub) * m.rhi : (int64_t)n * m.rlo;
+  }
+  template <bool TWO>
+  __device__ __forceinline__ static void load_rows(C (&v)[E::PV], const C* p, const ColMap& m, int t) {
+#pragma unroll
+    for (int q = 0; q < E::NB0; ++q) {
+      if (!E::live0(t, q)) continue;
+#pragma unroll
+      for (int r = 0; r < E::R0; ++r) v[q * E::R0 + r] = ld_stream(p + row_off<TWO>(m, E::in_index(t, q, r)));
+    }
+  }
+  template <bool TWO>
+  __device__ __forceinline__ static void store_rows(C (&v)[E::PV], C* p, const ColMap& m, int t, T sc,
+                                                    bool scaled) {
+#pragma unroll
+    for (int q = 0; q < E::NBL; ++q) {
+      if (!E::liveL(t, q)) continue;
+#pragma unroll
+      for (int r = 0; r < E::RL; ++r) {
+        C z = v[q * E::RL + r];
+        if (scaled) z = mk<T>(z.x * sc, z.y * sc);
+        st_stream(p + row_off<TWO>(m, E::out_index(t, q, r)), z);
+      }
+    }
+  }
+  // single-stride row maps on both sides (nsub >= N), all columns present
+  __device__ __forceinline__ static void load_full(C (&v)[E::PV], const C* in, const ColGeom& g, uint32_t tile,
+                                                   int c, int t) {
+    load_rows<false>(v, in + map_base(g.in, tile * CW + c), g.in, t);
+  }
+  __device__ __forceinline__ static void store_full(C (&v)[E::PV], C* out, const ColGeom& g, uint32_t tile,
+                                                    int c, int t) {
+    store_rows<false>(v, out + map_base(g.out, tile * CW + c), g.out, t, (T)g.scale, g.scale != 1.0);
+  }
+
   __device__ __forceinline__ static void tile(const C* in, C* out, const ColGeom& g, const C* __restrict__ tw,
                                               uint32_t tile_id, C* smem) {
     const int c = threadIdx.x % CW, t = threadIdx.x / CW;
@@ -378,7 +420,13 @@ constexpr int twiddle_count() {
 // hoisted out of the tile loop and held in registers for the whole kernel:
 // cheaper for small tiles, but for 768/1024-point columns at 16-24 points
 // per thread that exceeds the register budget (spills, or half the waves).
-template <typename T, bool INV, typename SC, int P, int CW, int ORD, int RA = 0>
+//
+// FULL: every tile has CW existing columns (ncols % CW == 0) and both row
+// maps have a single stride.  The tile loop then runs straight-line loads
+// and stores, and its last iteration (no prefetch) is peeled off, so the
+// wait before a tile's first use of its prefetched rows does not also wait
+// for the previous tile's stores.
+template <typename T, bool INV, typename SC, int P, int CW, int ORD, int RA = 0, bool FULL = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPipe<T, INV, SC, P, CW>::NT))) void k_cols_pipe(
     const cx<T>* in, cx<T>* out, ColGeom g, const cx<T>* tw, uint32_t ntiles) {
   using K_ = ColsPipe<T, INV, SC, P, CW>;
@@ -389,6 +437,23 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, ColsPipe<T, INV, SC, P,
   copy_twiddles<SC>(stw, tw, K_::NT);
   const int c0 = threadIdx.x % CW, t0 = threadIdx.x / CW;
   cx<T> cur[K_::E::PV], nxt[K_::E::PV];
+  if constexpr (FULL) {
+    K_::load_full(nxt, in, g, walk.at(0), c0, t0);
+    for (uint32_t k = 0; k + 1 < walk.count; ++k) {
+      int c = c0, t = t0;
+      if constexpr (RA != 0) __asm__ volatile("" : "+v"(c), "+v"(t));
+      ColLds<T, CW> L{smem, c};
+#pragma unroll
+      for (int i = 0; i < K_::E::PV; ++i) cur[i] = nxt[i];
+      K_::load_full(nxt, in, g, walk.at(k + 1), c, t);
+      K_::E::run(cur, t, L, stw);
+      K_::store_full(cur, out, g, walk.at(k), c, t);
+    }
+    ColLds<T, CW> L{smem, c0};
+    K_::E::run(nxt, t0, L, stw);
+    K_::store_full(nxt, out, g, walk.at(walk.count - 1), c0, t0);
+    return;
+  }
   K_::load(nxt, in, g, walk.at(0), c0, t0);
   for (uint32_t k = 0; k < walk.count; ++k) {
     int c = c0, t = t0;
@@ -494,14 +559,21 @@ inline int64_t persistent_blocks(int64_t ntiles, int occ) {
   return blocks < ntiles ? blocks : ntiles;
 }
 
-template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD, int RA = 0>
+// FL 1: the straight-line FULL kernel where the geometry allows it (else the
+// general one); FL 0: always the general kernel.
+template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD, int RA = 0, int FL = 0>
 hipError_t launch_cols_pipe(const void* in, void* out, ColGeom g, const void* tw, hipStream_t s) {
   const int64_t ntiles = ((int64_t)g.ncols + CW - 1) / CW;
   if (ntiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_cols_pipe<T, INV, SC, P, CW, ORD, RA>), dim3((uint32_t)persistent_blocks(ntiles, OCC)),
-                     dim3(ColsPipe<T, INV, SC, P, CW>::NT), 0, s, static_cast<const cx<T>*>(in),
-                     static_cast<cx<T>*>(out), g,
-                     static_cast<const cx<T>*>(tw), (uint32_t)ntiles);
+  const dim3 grid((uint32_t)persistent_blocks(ntiles, OCC)), block(ColsPipe<T, INV, SC, P, CW>::NT);
+  if (FL && g.ncols % CW == 0 && g.in.nsub >= SC::N && g.out.nsub >= SC::N)
+    hipLaunchKernelGGL((k_cols_pipe<T, INV, SC, P, CW, ORD, RA, true>), grid, block, 0, s,
+                       static_cast<const cx<T>*>(in), static_cast<cx<T>*>(out), g, static_cast<const cx<T>*>(tw),
+                       (uint32_t)ntiles);
+  else
+    hipLaunchKernelGGL((k_cols_pipe<T, INV, SC, P, CW, ORD, RA, false>), grid, block, 0, s,
+                       static_cast<const cx<T>*>(in), static_cast<cx<T>*>(out), g, static_cast<const cx<T>*>(tw),
+                       (uint32_t)ntiles);
   return hipGetLastError();
 }
 
@@ -557,7 +629,7 @@ KernelEntry entry_cols() {
                      reinterpret_cast<void*>(&launch_cols<T, INV, SC, P, CW, ORD>), 0, ORD};
 }
 
-template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD, int RA = 0>
+template <typename T, bool INV, typename SC, int P, int CW, int OCC, int ORD, int RA = 0, int FL = 0>
 KernelEntry entry_cols_pipe() {
   return KernelEntry{prec_of<T>(),
                      INV ? COLS_INV : COLS_FWD,
@@ -566,10 +638,11 @@ KernelEntry entry_cols_pipe() {
                      P,
                      CW,
                      ColsPipe<T, INV, SC, P, CW>::NT,
-                     reinterpret_cast<void*>(&launch_cols_pipe<T, INV, SC, P, CW, OCC, ORD, RA>),
+                     reinterpret_cast<void*>(&launch_cols_pipe<T, INV, SC, P, CW, OCC, ORD, RA, FL>),
                      OCC,
                      ORD,
-                     RA};
+                     RA,
+                     FL};
 }
 
 template <typename T, typename SC, int P, int G, int OCC, int ORD>

@@ -107,6 +107,7 @@ std::vector<std::pair<const KernelEntry*, const KernelEntry*>> find_pairs(int pr
     for (int j = 0; j < cnt; ++j)
       if (tab[j].kind == ki && tab[j].N == n && tab[j].P == tab[i].P && tab[j].tile == tab[i].tile &&
           tab[j].persist == tab[i].persist && tab[j].order == tab[i].order && tab[j].readdr == tab[i].readdr &&
+          tab[j].straight == tab[i].straight &&
           same_sched(tab[j].sched, tab[i].sched)) {
         out.emplace_back(&tab[i], &tab[j]);
         break;
@@ -218,11 +219,11 @@ std::string axis_json(const char* name, const AxisPlan& a) {
   std::snprintf(buf, sizeof buf,
                 "\"%s\":{\"n\":%d,\"pass\":\"%s\",\"engine\":\"%s\",\"radices\":%s,\"points_per_thread\":%d,"
                 "\"tile\":%d,\"threads\":%d,\"twiddle_bytes\":%zu,\"variant\":%d,\"candidates\":%d,"
-                "\"persist\":%d,\"order\":%d,\"readdr\":%d,\"tuned_ms\":%.6f}",
+                "\"persist\":%d,\"order\":%d,\"readdr\":%d,\"straight\":%d,\"tuned_ms\":%.6f}",
                 name, a.N, a.rows ? "rows_two_for_one" : "strided_columns", e ? "register" : "generic",
                 sched_str(a.sched).c_str(), e ? e->P : 0, e ? e->tile : 0, e ? e->threads : 256, a.tw_bytes,
                 a.variant, (int)a.cands.size(), e ? e->persist : 0, e ? e->order : 0, e ? e->readdr : 0,
-                a.tuned_ms);
+                e ? e->straight : 0, a.tuned_ms);
   return buf;
 }
 
@@ -572,8 +573,8 @@ std::string cand_desc(const AxisPlan& a, int v) {
   std::string r;
   for (int i = 0; i < e->sched.S; ++i) r += (i ? "." : "") + std::to_string(e->sched.R[i]);
   char b[160];
-  std::snprintf(b, sizeof b, "P%d:T%d:W%d:K%d:O%d:A%d:R%s", e->P, e->tile, e->threads, e->persist, e->order,
-                e->readdr, r.c_str());
+  std::snprintf(b, sizeof b, "P%d:T%d:W%d:K%d:O%d:A%d:S%d:R%s", e->P, e->tile, e->threads, e->persist, e->order,
+                e->readdr, e->straight, r.c_str());
   return b;
 }
 }  // namespace

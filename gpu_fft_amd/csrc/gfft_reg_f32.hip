@@ -100,6 +100,13 @@ int registry_f32(const KernelEntry** out) {
       GFFT_COLS_PIPE(float, Sc512, 8, 16, 2, 1),
       GFFT_COLS_PIPE(float, Sc512, 16, 16, 2, 1),
       GFFT_COLS_PIPE(float, Sc512, 32, 16, 2, 1),
+      GFFT_COLS_PIPE_F(float, Sc512, 16, 16, 1, 0, 0),
+      GFFT_COLS_PIPE_F(float, Sc512, 16, 16, 1, 1, 0),
+      GFFT_COLS_PIPE_F(float, Sc768, 24, 16, 1, 0, 1),
+      GFFT_COLS_PIPE_F(float, Sc768, 24, 16, 1, 1, 1),
+      GFFT_COLS_PIPE_F(float, Sc1024, 16, 16, 1, 0, 1),
+      GFFT_COLS_PIPE_F(float, Sc1024, 16, 16, 1, 1, 1),
+      GFFT_COLS_PIPE_F(float, Sc1024c, 16, 16, 1, 0, 1),
       GFFT_ROWS_PIPE(float, Sc512, 8, 4, 2, 0),
       GFFT_ROWS_PIPE(float, Sc512, 8, 4, 4, 0),
       GFFT_ROWS_PIPE(float, Sc512, 8, 4, 2, 1),

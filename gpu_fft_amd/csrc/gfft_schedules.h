@@ -99,6 +99,10 @@ using Sc832 = Sched<8, 8, 13>;
 // the same with per-tile recomputed element offsets (RA 1 in k_cols_pipe)
 #define GFFT_COLS_PIPE_RA(T, SC, PC, CW, OCC, ORD) \
   entry_cols_pipe<T, false, SC, PC, CW, OCC, ORD, 1>(), entry_cols_pipe<T, true, SC, PC, CW, OCC, ORD, 1>()
+// the same with the straight-line loop for full tiles (FL 1 in launch_cols_pipe)
+#define GFFT_COLS_PIPE_F(T, SC, PC, CW, OCC, ORD, RA)                 \
+  entry_cols_pipe<T, false, SC, PC, CW, OCC, ORD, RA, 1>(),          \
+      entry_cols_pipe<T, true, SC, PC, CW, OCC, ORD, RA, 1>()
 // persistent software-pipelined z passes, G row pairs per tile
 #define GFFT_ROWS_PIPE(T, SC, PR, G, OCC, ORD) \
   entry_rows_r2c_pipe<T, SC, PR, G, OCC, ORD>(), entry_rows_c2r_pipe<T, SC, PR, G, OCC, ORD>()
