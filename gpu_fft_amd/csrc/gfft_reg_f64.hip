@@ -40,6 +40,11 @@ int registry_f64(const KernelEntry** out) {
       GFFT_COLS_PIPE(double, Sc512, 16, 8, 2, 1),
       GFFT_ROWS_PIPE(double, Sc512, 8, 4, 2, 1),
       GFFT_ROWS_PIPE(double, Sc512, 8, 2, 4, 1),
+      GFFT_COLS_PIPE_F(double, Sc512, 16, 8, 1, 0, 0),
+      GFFT_COLS_PIPE_F(double, Sc512, 16, 8, 1, 1, 0),
+      GFFT_COLS_PIPE_F(double, Sc512, 8, 8, 2, 0, 0),
+      GFFT_COLS_PIPE_F(double, Sc512, 16, 8, 2, 0, 1),
+      GFFT_COLS_PIPE_F(double, Sc512, 16, 8, 2, 1, 1),
       GFFT_COLS_PIPE_RA(double, Sc1024, 16, 8, 1, 0),
       GFFT_COLS_PIPE_RA(double, Sc1024, 16, 8, 1, 1),
       GFFT_COLS_PIPE_RA(double, Sc1024, 16, 4, 2, 0),
