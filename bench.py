@@ -486,7 +486,7 @@ def main():
     if not args.no_tune:
         # plan-time autotuning (outside the timed region): it overwrites the
         # buffer, which is regenerated from the same seed afterwards
-        plan.tune(buf, reps=3)
+        plan.tune(buf, reps=10)
         torch.cuda.synchronize()
         gen.manual_seed(0)
         buf.copy_(torch.rand((nx, ny, 2 * K), dtype=dt, device=dev, generator=gen) * 2 - 1)
