@@ -251,7 +251,7 @@ def pct(xs, q):
 
 
 def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: float = 64.0, nstreams: int = 4,
-                seed: int = 0) -> dict:
+                seed: int = 0, small_lane: int = -1) -> dict:
     """Poisson-arrival serving run of the config-5 mix.
 
     1. Plans of all 84 (size, precision) geometries are built up front
@@ -265,7 +265,9 @@ def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: flo
     geometry (requests of one geometry share it; each request carries the
     unitary scale 1/sqrt(N) so repeated transforms keep the values bounded).
     Host-staged requests copy from / to one pinned host array per precision
-    through the server's budget-sized arena."""
+    through the server's budget-sized arena.  small_lane >= 0 sets the
+    reserved small-request stream's threshold in bytes (0 = no reservation);
+    -1 keeps the server's default."""
     import torch
     from gpu_fft_amd import serving as SV
     dev = torch.device("cuda:0")
@@ -273,6 +275,8 @@ def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: flo
     reqs = serve_dims(n_requests, seed)
     keys = sorted({(n, p) for n, p, _ in reqs + reqs_cap})
     srv = SV.Server(int(budget_gb * (1 << 30)), nstreams=nstreams, staging=staged)
+    if small_lane >= 0:
+        srv.set_small_lane(small_lane)
     for n, p in keys:
         srv.prepare(n, n, n, p)
     bufs, hin, hout = {}, {}, {}
@@ -333,6 +337,9 @@ def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: flo
     bytes_total = sum(r["bytes"] for r in ok)
     flops = sum(5.0 * n ** 3 * math.log2(n ** 3) for n, _, _ in reqs)
     by_prec = {p: [(r["done"] - r["arrival"]) * 1e3 for r in ok if reqs[r["tag"]][1] == p] for p in ("f32", "f64")}
+    buckets = (("<=128", 0, 128), ("129-256", 129, 256), (">256", 257, 1 << 30))
+    by_edge = {name: [(r["done"] - r["arrival"]) * 1e3 for r in ok if lo <= reqs[r["tag"]][0] <= hi]
+               for name, lo, hi in buckets}
     return {
         "mode": "host-staged (H2D + transform + D2H)" if staged else "device-resident, in place",
         "requests": len(reqs), "failed": len(res) - len(ok),
@@ -346,11 +353,14 @@ def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: flo
         if by_prec["f32"] else None,
         "latency_ms_f64": {"p50": round(pct(by_prec["f64"], 50), 4), "p99": round(pct(by_prec["f64"], 99), 4)}
         if by_prec["f64"] else None,
+        "latency_ms_by_edge": {k: {"n": len(v), "p50": round(pct(v, 50), 4), "p99": round(pct(v, 99), 4)}
+                               for k, v in by_edge.items() if v},
         "queue_ms": {"p50": round(pct(queue, 50), 4), "p99": round(pct(queue, 99), 4)},
         "gpu_ms_mean": round(sum(r["gpu_ms"] for r in ok) / len(ok), 4),
         "staged_ms_mean": round(sum(r["staged_ms"] for r in ok) / len(ok), 4),
         "budget_bytes": st["budget"], "peak_in_flight_bytes": st["peak_in_flight_bytes"],
         "plans_resident": st["plans"], "plan_bytes": st["plan_bytes"], "streams": nstreams,
+        "small_lane_bytes": small_lane if small_lane >= 0 else (32 << 20 if nstreams >= 2 else 0),
         "max_submit_lateness_ms": round(late * 1e3, 4),
         "workload": "edges log-uniform in [64,512] -> nearest of 42 sizes 2^a3^b5^c, 80/20 FP32/FP64, "
                     "R2C/C2R 50/50, Poisson arrivals, seed 0",
@@ -358,8 +368,9 @@ def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: flo
 
 
 def serve_main(args) -> dict:
-    dev_res = serve_bench(args.serve, staged=False, load=args.serve_load)
-    host_res = serve_bench(max(100, args.serve // 5), staged=True, load=args.serve_load)
+    dev_res = serve_bench(args.serve, staged=False, load=args.serve_load, small_lane=args.serve_small_lane)
+    host_res = serve_bench(max(100, args.serve // 5), staged=True, load=args.serve_load,
+                           small_lane=args.serve_small_lane)
     return {"metric": "served mixed-size R2C/C2R requests: p50/p99 arrival-to-completion latency at "
                       f"{args.serve_load:.0%} of capacity",
             "value": dev_res["latency_ms"]["p99"], "unit": "ms (p99 latency, device-resident requests)",
@@ -425,6 +436,8 @@ def main():
     ap.add_argument("--serve", type=int, default=0, metavar="REQUESTS",
                     help="serving mode: REQUESTS Poisson arrivals of the mixed-size request stream")
     ap.add_argument("--serve-load", type=float, default=0.8, help="offered load as a fraction of capacity")
+    ap.add_argument("--serve-small-lane", type=int, default=-1, metavar="BYTES",
+                    help="reserved small-request stream threshold (0 = off, -1 = server default)")
     ap.add_argument("--accuracy", action="store_true",
                     help="relative L2 error vs the FP64 oracle at every config grid (no timing)")
     ap.add_argument("--no-extras", action="store_true",

@@ -39,6 +39,10 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// default bound of the reserved stream's requests: 32 MiB (a 200^3 FP32 or a
+// 160^3 FP64 array)
+constexpr size_t kDefaultSmallBytes = size_t(32) << 20;
+
 double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
 
 struct Events {
@@ -88,6 +92,9 @@ struct gfft_server_s {
   std::vector<hipStream_t> streams;
   std::vector<Lane> lanes;  // one per stream
   size_t n_running = 0;
+  // requests of at most this many bytes may use the last stream, which no
+  // larger request uses (0: no reserved stream)
+  size_t small_bytes = 0;
   std::map<Key, std::unique_ptr<gfft::PlanCore>> plans;
   std::map<Key, int> checked;  // host-side plan status of geometries seen at submission
   size_t plan_bytes = 0;
@@ -222,10 +229,17 @@ void admit(gfft_server_s* s) {
         q->e = s->ev_pool[q->ev];
     }
     if (st == GFFT_OK) {
-      // the stream with the least outstanding work (array bytes)
-      int best = 0;
-      for (int i = 1; i < (int)s->lanes.size(); ++i)
-        if (s->lanes[i].bytes < s->lanes[best].bytes) best = i;
+      // the stream with the least outstanding work (array bytes); with a
+      // reserved stream, larger requests never queue on it, so small ones
+      // always have a stream without a large request ahead of them
+      const int nl = (int)s->lanes.size();
+      const bool reserve = s->small_bytes > 0 && nl >= 2;
+      const bool small = reserve && q->bytes <= s->small_bytes;
+      int best = -1;
+      for (int i = 0; i < nl; ++i) {
+        if (reserve && !small && i == nl - 1) continue;
+        if (best < 0 || s->lanes[i].bytes < s->lanes[best].bytes) best = i;
+      }
       q->stream = best;
       s->lanes[best].bytes += q->bytes;
       st = launch(s, q);
@@ -313,6 +327,13 @@ int gfft_request_bytes(int64_t nx, int64_t ny, int64_t nz, int prec, size_t* byt
   return request_bytes(nx, ny, nz, prec, bytes);
 }
 
+int gfft_server_set_small_lane(gfft_server s, size_t max_bytes) {
+  if (!s) return GFFT_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->small_bytes = max_bytes;
+  return GFFT_OK;
+}
+
 int gfft_server_create(gfft_server* out, size_t budget_bytes, int nstreams, int flags) {
   if (!out || budget_bytes == 0 || nstreams < 1 || nstreams > 64 || (flags & ~GFFT_SERVER_STAGING))
     return GFFT_ERR_INVALID_ARG;
@@ -320,6 +341,7 @@ int gfft_server_create(gfft_server* out, size_t budget_bytes, int nstreams, int 
   gfft_server_s* s = new (std::nothrow) gfft_server_s;
   if (!s) return GFFT_ERR_OUT_OF_MEMORY;
   s->flags = flags;
+  s->small_bytes = nstreams >= 2 ? kDefaultSmallBytes : 0;
   s->adm = gfft::Admission<Req*>(budget_bytes);
   int st = hipGetDevice(&s->device) == hipSuccess ? GFFT_OK : GFFT_ERR_HIP;
   if (st == GFFT_OK && (flags & GFFT_SERVER_STAGING)) {

@@ -70,6 +70,8 @@ def _bind(L):
     L.gfft_request_bytes.argtypes = [i64, i64, i64, ctypes.c_int, ctypes.POINTER(sz)]
     L.gfft_server_create.restype = ctypes.c_int
     L.gfft_server_create.argtypes = [ctypes.POINTER(vp), sz, ctypes.c_int, ctypes.c_int]
+    L.gfft_server_set_small_lane.restype = ctypes.c_int
+    L.gfft_server_set_small_lane.argtypes = [vp, sz]
     L.gfft_server_prepare.restype = ctypes.c_int
     L.gfft_server_prepare.argtypes = [vp, i64, i64, i64, ctypes.c_int]
     L.gfft_server_submit.restype = ctypes.c_int
@@ -118,6 +120,10 @@ class Server:
         if self._h is None:
             raise ValueError("server destroyed")
         return self._h
+
+    def set_small_lane(self, max_bytes: int) -> None:
+        """Reserve the last stream for requests of at most max_bytes (0 = off)."""
+        _check(lib().gfft_server_set_small_lane(self.handle, int(max_bytes)), "gfft_server_set_small_lane")
 
     def prepare(self, nx, ny, nz, precision="f32") -> None:
         _check(lib().gfft_server_prepare(self.handle, nx, ny, nz, _prec(precision)), "gfft_server_prepare")

@@ -307,6 +307,13 @@ int gfft_request_bytes(int64_t nx, int64_t ny, int64_t nz, int prec, size_t* byt
  * GFFT_SERVER_STAGING.  Uses the current HIP device. */
 int gfft_server_create(gfft_server* srv, size_t budget_bytes, int nstreams, int flags);
 
+/* Reserve the last stream for requests of at most max_bytes (array bytes):
+ * larger requests never queue there, so a small request is never stuck
+ * behind a large one on every stream.  Default with nstreams >= 2: 32 MiB;
+ * 0 turns the reservation off (every request goes to the stream with the
+ * least outstanding bytes).  Applies to requests admitted afterwards. */
+int gfft_server_set_small_lane(gfft_server srv, size_t max_bytes);
+
 /* Build and keep the plan of a geometry now (otherwise built on first use,
  * which allocates and synchronises). */
 int gfft_server_prepare(gfft_server srv, int64_t nx, int64_t ny, int64_t nz, int prec);

@@ -70,6 +70,7 @@ def test_null_arguments_rejected_without_gpu():
     assert L.gfft_exec_c2r(None, None, None) == 1
     v = ctypes.c_size_t()
     assert L.gfft_plan_device_bytes(None, ctypes.byref(v)) == 1
+    assert L.gfft_server_set_small_lane(None, ctypes.c_size_t(1)) == 1
 
 
 def test_plan_choice_512_uses_register_kernels():

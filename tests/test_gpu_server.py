@@ -159,3 +159,29 @@ def test_request_scale_normalises_round_trip():
     assert all(r["status"] == 0 for r in res)
     err = O.rel_l2(y.cpu().numpy()[..., :75], x.cpu().numpy()[..., :75])
     assert err <= 1e-13, err
+
+
+def test_small_lane_keeps_large_requests_off_the_last_stream():
+    """The reserved stream only ever runs requests at or under the threshold;
+    results stay bitwise those of the plan API; threshold 0 lifts it."""
+    import torch
+    big, small = ((256, 256, 256), "f32"), ((64, 64, 64), "f64")
+    limit = S.request_bytes(*small[0], small[1])
+    assert S.request_bytes(*big[0], big[1]) > limit
+    reqs = [big if i % 3 == 0 else small for i in range(18)]
+    say(f"case small lane, {len(reqs)} requests, limit {limit} bytes")
+    inputs = [make_input(torch, *d, p, S.R2C, seed=90 + i) for i, (d, p) in enumerate(reqs)]
+    work = [x.clone() for x in inputs]
+    with S.Server(budget_bytes=8 << 30, nstreams=3) as srv:
+        srv.set_small_lane(limit)
+        for i, ((d, p), w) in enumerate(zip(reqs, work)):
+            srv.submit(d, p, S.R2C, w, tag=i)
+        res = srv.drain(timeout_s=120)
+        srv.set_small_lane(0)
+    assert len(res) == len(reqs) and all(r["status"] == 0 for r in res)
+    for r in res:
+        if r["bytes"] > limit:
+            assert r["stream"] in (0, 1), r
+    assert any(r["stream"] == 2 for r in res)
+    for (d, p), x, w in zip(reqs, inputs, work):
+        assert torch.equal(w, alone(torch, d, p, S.R2C, x)), (d, p)
