@@ -335,7 +335,8 @@ int gfft_server_set_small_lane(gfft_server s, size_t max_bytes) {
 }
 
 int gfft_server_create(gfft_server* out, size_t budget_bytes, int nstreams, int flags) {
-  if (!out || budget_bytes == 0 || nstreams < 1 || nstreams > 64 || (flags & ~GFFT_SERVER_STAGING))
+  if (!out || budget_bytes == 0 || nstreams < 1 || nstreams > 64 ||
+      (flags & ~(GFFT_SERVER_STAGING | GFFT_SERVER_SHARED_QUEUES)))
     return GFFT_ERR_INVALID_ARG;
   *out = nullptr;
   gfft_server_s* s = new (std::nothrow) gfft_server_s;
@@ -353,9 +354,27 @@ int gfft_server_create(gfft_server* out, size_t budget_bytes, int nstreams, int 
       s->arena.reset(budget_bytes);
     }
   }
+  // Streams with a CU mask (every CU enabled) get a hardware queue each.
+  // Plain streams come from the runtime's pool of a few queues; once the
+  // process holds more streams than the pool has queues (or has created and
+  // destroyed some), two server streams can land on one queue, and a small
+  // request then waits behind the large one on the shared queue.
+  std::vector<uint32_t> cu_mask;
+  if (st == GFFT_OK && !(flags & GFFT_SERVER_SHARED_QUEUES)) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || ncu < 1) {
+      st = GFFT_ERR_HIP;
+    } else {
+      cu_mask.assign((size_t)(ncu + 31) / 32, 0xffffffffu);
+      if (ncu % 32) cu_mask.back() = (1u << (ncu % 32)) - 1u;
+    }
+  }
   for (int i = 0; i < nstreams && st == GFFT_OK; ++i) {
     hipStream_t h = nullptr;
-    if (hipStreamCreateWithFlags(&h, hipStreamNonBlocking) != hipSuccess) {
+    const hipError_t e = cu_mask.empty()
+                             ? hipStreamCreateWithFlags(&h, hipStreamNonBlocking)
+                             : hipExtStreamCreateWithCUMask(&h, (uint32_t)cu_mask.size(), cu_mask.data());
+    if (e != hipSuccess) {
       st = GFFT_ERR_HIP;
       break;
     }

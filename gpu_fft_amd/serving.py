@@ -21,6 +21,7 @@ __all__ = ["Server", "Request", "Result", "clock", "request_bytes", "R2C", "C2R"
 
 R2C, C2R = 0, 1
 STAGING = 1
+SHARED_QUEUES = 2
 REQ_DEVICE = 1
 
 def _is_235(n: int) -> bool:
@@ -107,11 +108,11 @@ def _ptr(x) -> int:
 class Server:
     """A serving runtime on the current HIP device."""
 
-    def __init__(self, budget_bytes: int, nstreams: int = 4, staging: bool = False):
+    def __init__(self, budget_bytes: int, nstreams: int = 4, staging: bool = False, own_queues: bool = True):
         L = _bind(lib())
         h = ctypes.c_void_p()
-        _check(L.gfft_server_create(ctypes.byref(h), int(budget_bytes), int(nstreams), STAGING if staging else 0),
-               "gfft_server_create")
+        flags = (STAGING if staging else 0) | (0 if own_queues else SHARED_QUEUES)
+        _check(L.gfft_server_create(ctypes.byref(h), int(budget_bytes), int(nstreams), flags), "gfft_server_create")
         self._h = h
         self._keep = {}  # request id -> tensors that must outlive the request
 
@@ -190,4 +191,4 @@ class Server:
             pass
 
 
-__all__ += ["F32", "F64", "GfftError", "STAGING", "REQ_DEVICE"]
+__all__ += ["F32", "F64", "GfftError", "STAGING", "SHARED_QUEUES", "REQ_DEVICE"]

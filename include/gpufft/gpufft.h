@@ -267,6 +267,7 @@ int gfft_batch_destroy(gfft_batch batch);
 typedef struct gfft_server_s* gfft_server;
 
 #define GFFT_SERVER_STAGING 1 /* create: allocate a budget-sized arena for host-staged requests */
+#define GFFT_SERVER_SHARED_QUEUES 2 /* create: plain non-blocking streams from the runtime's queue pool */
 #define GFFT_REQ_DEVICE 1     /* request: data is device memory, transformed in place */
 
 typedef struct {
@@ -304,7 +305,12 @@ double gfft_server_clock(void);
 int gfft_request_bytes(int64_t nx, int64_t ny, int64_t nz, int prec, size_t* bytes);
 
 /* budget_bytes bounds the arrays of admitted, unfinished requests; flags:
- * GFFT_SERVER_STAGING.  Uses the current HIP device. */
+ * GFFT_SERVER_STAGING, GFFT_SERVER_SHARED_QUEUES.  Uses the current HIP
+ * device.  By default each of the nstreams streams is created with a CU mask
+ * of every CU, which gives it a hardware queue of its own (the streams then
+ * synchronise with the legacy null stream, like any blocking stream); with
+ * GFFT_SERVER_SHARED_QUEUES they are plain non-blocking streams, which the
+ * runtime may map several to one hardware queue. */
 int gfft_server_create(gfft_server* srv, size_t budget_bytes, int nstreams, int flags);
 
 /* Reserve the last stream for requests of at most max_bytes (array bytes):

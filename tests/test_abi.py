@@ -71,6 +71,8 @@ def test_null_arguments_rejected_without_gpu():
     v = ctypes.c_size_t()
     assert L.gfft_plan_device_bytes(None, ctypes.byref(v)) == 1
     assert L.gfft_server_set_small_lane(None, ctypes.c_size_t(1)) == 1
+    h = ctypes.c_void_p()
+    assert L.gfft_server_create(ctypes.byref(h), ctypes.c_size_t(1 << 20), 2, 4) == 1  # unknown flag
 
 
 def test_plan_choice_512_uses_register_kernels():

@@ -185,3 +185,23 @@ def test_small_lane_keeps_large_requests_off_the_last_stream():
     assert any(r["stream"] == 2 for r in res)
     for (d, p), x, w in zip(reqs, inputs, work):
         assert torch.equal(w, alone(torch, d, p, S.R2C, x)), (d, p)
+
+
+def test_shared_queue_streams_give_the_same_results():
+    """GFFT_SERVER_SHARED_QUEUES (plain streams from the runtime's pool)
+    against the default CU-masked streams with queues of their own."""
+    import torch
+    reqs = mix(12, seed=4, max_edge=128)
+    say(f"case shared vs own queues, {len(reqs)} requests")
+    inputs = [make_input(torch, *d, p, dr, seed=120 + i) for i, (d, p, dr) in enumerate(reqs)]
+    out = {}
+    for own in (True, False):
+        work = [x.clone() for x in inputs]
+        with S.Server(budget_bytes=2 << 30, nstreams=4, own_queues=own) as srv:
+            for (d, p, dr), w in zip(reqs, work):
+                srv.submit(d, p, dr, w)
+            res = srv.drain(timeout_s=120)
+        assert len(res) == len(reqs) and all(r["status"] == 0 for r in res)
+        out[own] = work
+    for a, b in zip(out[True], out[False]):
+        assert torch.equal(a, b)
