@@ -569,8 +569,8 @@ def main():
     for j, nm in enumerate(names):
         ts = [evs[k][j][0].elapsed_time(evs[k][j][1]) for k in range(args.steps)]
         mean = sum(ts) / len(ts)
-        per_pass[nm] = {"ms_mean": round(mean, 5), "ms_min": round(min(ts), 5),
-                        "GBps": round(2 * S / (mean * 1e-3) / 1e9, 1)}
+        per_pass[nm] = {"ms_mean": round(mean, 5), "ms_median": round(statistics.median(ts), 5),
+                        "ms_min": round(min(ts), 5), "GBps": round(2 * S / (mean * 1e-3) / 1e9, 1)}
     dom = max(per_pass, key=lambda k: per_pass[k]["ms_mean"])
     dom_ms = per_pass[dom]["ms_mean"]
     achieved = 2 * S / (dom_ms * 1e-3) / 1e9
@@ -609,6 +609,9 @@ def main():
                                 "compulsory_frac": round(4 * S / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
         "per_pass": per_pass,
         "ms_per_step_per_pass_region": round(ms_step_passes, 5),
+        # the passes of each direction (R2C: the first half of `passes`)
+        "r2c_ms": round(sum(per_pass[nm]["ms_mean"] for nm in names[:len(names) // 2]), 5),
+        "c2r_ms": round(sum(per_pass[nm]["ms_mean"] for nm in names[len(names) // 2:]), 5),
         "variants": {ax: {k: desc[ax][k] for k in ("engine", "radices", "points_per_thread", "tile", "threads",
                                                    "persist", "order", "readdr", "straight", "variant")} for ax in ("z", "y", "x")},
         "zy_fused": int(desc.get("zy_fused", 0)),
