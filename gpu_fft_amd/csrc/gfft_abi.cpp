@@ -1,4 +1,5 @@
 // extern "C" entry points declared in include/gpufft/gpufft.h.
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <string>
@@ -19,6 +20,12 @@ int copy_text(const std::string& s, char* buf, size_t len) {
   return GFFT_OK;
 }
 
+// every kernel moves whole complex elements: 8 B (FP32) or 16 B (FP64)
+bool misaligned(const gfft_plan_s* p, const void* ptr) {
+  const uintptr_t a = p->core.prec == gfft::F32 ? 8 : 16;
+  return (reinterpret_cast<uintptr_t>(ptr) & (a - 1)) != 0;
+}
+
 int check_device(const gfft_plan_s* p) {
   int dev = -1;
   if (hipGetDevice(&dev) != hipSuccess) return GFFT_ERR_HIP;
@@ -34,7 +41,8 @@ int gfft_version(void) { return GFFT_VERSION; }
 const char* gfft_strerror(int status) {
   switch (status) {
     case GFFT_OK: return "success";
-    case GFFT_ERR_INVALID_ARG: return "invalid argument (null handle or pointer, or bad precision flag)";
+    case GFFT_ERR_INVALID_ARG:
+      return "invalid argument (null handle or pointer, data pointer not aligned to a complex element, or bad flag)";
     case GFFT_ERR_INVALID_DIMS: return "invalid dimensions (an edge < 1, procs does not divide Nx and Ny, or overflow)";
     case GFFT_ERR_UNSUPPORTED_SIZE: return "unsupported edge length (prime factor > 13 or above the LDS-resident limit)";
     case GFFT_ERR_UNSUPPORTED_PROCS: return "full transform with procs > 1 needs an inter-rank exchange (not provided)";
@@ -81,7 +89,7 @@ int gfft_plan_destroy(gfft_plan plan) {
 }
 
 int gfft_exec_r2c(gfft_plan plan, void* data, gfft_stream_t stream) {
-  if (!plan || !data) return GFFT_ERR_INVALID_ARG;
+  if (!plan || !data || misaligned(plan, data)) return GFFT_ERR_INVALID_ARG;
   if (plan->core.procs != 1) return GFFT_ERR_UNSUPPORTED_PROCS;
   const int st = check_device(plan);
   if (st != GFFT_OK) return st;
@@ -89,7 +97,7 @@ int gfft_exec_r2c(gfft_plan plan, void* data, gfft_stream_t stream) {
 }
 
 int gfft_exec_c2r(gfft_plan plan, void* data, gfft_stream_t stream) {
-  if (!plan || !data) return GFFT_ERR_INVALID_ARG;
+  if (!plan || !data || misaligned(plan, data)) return GFFT_ERR_INVALID_ARG;
   if (plan->core.procs != 1) return GFFT_ERR_UNSUPPORTED_PROCS;
   const int st = check_device(plan);
   if (st != GFFT_OK) return st;
@@ -97,35 +105,39 @@ int gfft_exec_c2r(gfft_plan plan, void* data, gfft_stream_t stream) {
 }
 
 int gfft_exec_axis(gfft_plan plan, int axis, int inverse, void* data, gfft_stream_t stream) {
-  if (!plan || !data || axis < 0 || axis > 3) return GFFT_ERR_INVALID_ARG;
+  if (!plan || !data || axis < 0 || axis > 3 || misaligned(plan, data)) return GFFT_ERR_INVALID_ARG;
   const int st = check_device(plan);
   if (st != GFFT_OK) return st;
   return gfft::exec_axis(plan->core, axis, inverse != 0, data, reinterpret_cast<hipStream_t>(stream));
 }
 
 int gfft_exec_r2c_local(gfft_plan plan, void* data, void* sendbuf, gfft_stream_t stream) {
-  if (!plan || !data || !sendbuf || data == sendbuf) return GFFT_ERR_INVALID_ARG;
+  if (!plan || !data || !sendbuf || data == sendbuf || misaligned(plan, data) || misaligned(plan, sendbuf))
+    return GFFT_ERR_INVALID_ARG;
   const int st = check_device(plan);
   if (st != GFFT_OK) return st;
   return gfft::exec_r2c_local(plan->core, data, sendbuf, reinterpret_cast<hipStream_t>(stream));
 }
 
 int gfft_exec_r2c_finish(gfft_plan plan, const void* recvbuf, void* out, gfft_stream_t stream) {
-  if (!plan || !recvbuf || !out || recvbuf == out) return GFFT_ERR_INVALID_ARG;
+  if (!plan || !recvbuf || !out || recvbuf == out || misaligned(plan, recvbuf) || misaligned(plan, out))
+    return GFFT_ERR_INVALID_ARG;
   const int st = check_device(plan);
   if (st != GFFT_OK) return st;
   return gfft::exec_r2c_finish(plan->core, recvbuf, out, reinterpret_cast<hipStream_t>(stream));
 }
 
 int gfft_exec_c2r_start(gfft_plan plan, const void* data, void* sendbuf, gfft_stream_t stream) {
-  if (!plan || !data || !sendbuf || data == sendbuf) return GFFT_ERR_INVALID_ARG;
+  if (!plan || !data || !sendbuf || data == sendbuf || misaligned(plan, data) || misaligned(plan, sendbuf))
+    return GFFT_ERR_INVALID_ARG;
   const int st = check_device(plan);
   if (st != GFFT_OK) return st;
   return gfft::exec_c2r_start(plan->core, data, sendbuf, reinterpret_cast<hipStream_t>(stream));
 }
 
 int gfft_exec_c2r_local(gfft_plan plan, const void* recvbuf, void* data, gfft_stream_t stream) {
-  if (!plan || !recvbuf || !data || recvbuf == data) return GFFT_ERR_INVALID_ARG;
+  if (!plan || !recvbuf || !data || recvbuf == data || misaligned(plan, recvbuf) || misaligned(plan, data))
+    return GFFT_ERR_INVALID_ARG;
   const int st = check_device(plan);
   if (st != GFFT_OK) return st;
   return gfft::exec_c2r_local(plan->core, recvbuf, data, reinterpret_cast<hipStream_t>(stream));
@@ -137,7 +149,7 @@ int gfft_plan_set_variant(gfft_plan plan, int axis, int variant) {
 }
 
 int gfft_plan_tune(gfft_plan plan, void* scratch, gfft_stream_t stream, int reps) {
-  if (!plan || !scratch) return GFFT_ERR_INVALID_ARG;
+  if (!plan || !scratch || misaligned(plan, scratch)) return GFFT_ERR_INVALID_ARG;
   const int st = check_device(plan);
   if (st != GFFT_OK) return st;
   return gfft::tune(plan->core, scratch, reinterpret_cast<hipStream_t>(stream), reps);

@@ -606,8 +606,8 @@ int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec)
 
 int gfft_batch_set_data(gfft_batch b, void* const* ptrs) {
   if (!b || !ptrs) return GFFT_ERR_INVALID_ARG;
-  for (int i = 0; i < b->count; ++i)
-    if (!ptrs[i]) return GFFT_ERR_INVALID_ARG;
+  for (int i = 0; i < b->count; ++i)  // whole float2 elements: 8 B aligned
+    if (!ptrs[i] || (reinterpret_cast<uintptr_t>(ptrs[i]) & 7u) != 0) return GFFT_ERR_INVALID_ARG;
   for (int i = 0; i < b->count; ++i) b->hx[b->pos[i]].data = static_cast<float*>(ptrs[i]);
   const int st = upload_xf(b);  // synchronises the device first
   if (any_plain(b)) drop_graphs(b);

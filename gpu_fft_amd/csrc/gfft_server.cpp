@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdint>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -416,6 +417,9 @@ int gfft_server_submit(gfft_server s, const gfft_request* req, uint64_t* id) {
   size_t bytes = 0;
   int st = request_bytes(req->nx, req->ny, req->nz, req->prec, &bytes);
   if (st != GFFT_OK) return st;
+  // device arrays are transformed in place: whole complex elements (8 / 16 B)
+  if (device && (reinterpret_cast<uintptr_t>(req->data) & (req->prec == GFFT_F64 ? 15u : 7u)) != 0)
+    return GFFT_ERR_INVALID_ARG;
   {
     // sizes the planner cannot serve (a prime factor > 13, ...) fail here,
     // not at admission; the host-only check is cached per geometry

@@ -50,7 +50,7 @@ typedef enum {
 
 typedef enum {
   GFFT_OK = 0,
-  GFFT_ERR_INVALID_ARG = 1,       /* NULL handle/pointer, bad precision flag */
+  GFFT_ERR_INVALID_ARG = 1,       /* NULL handle/pointer, data not aligned to a complex element (8 B FP32, 16 B FP64), bad flag */
   GFFT_ERR_INVALID_DIMS = 2,      /* a dimension < 1, p does not divide Nx and Ny, sizes overflow */
   GFFT_ERR_UNSUPPORTED_SIZE = 3,  /* an edge has a prime factor > 13 or exceeds the LDS-resident limit */
   GFFT_ERR_UNSUPPORTED_PROCS = 4, /* full transform requested with procs > 1 (needs an exchange) */

@@ -161,6 +161,15 @@ def test_batch_rejects_unsupported_sizes():
         G.Batch([(32, 64, 32)])
 
 
+def test_batch_rejects_misaligned_arrays():
+    import torch
+    buf = torch.zeros((32, 32, 34), device="cuda:0")
+    with G.Batch([(32, 32, 32)]) as b:
+        with pytest.raises(G.GfftError) as e:
+            b.set_data([buf.data_ptr() + 4])
+        assert e.value.status == 1
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_every_variant_matches_oracle(variant):
     """Each compiled variant (radix order, pencils per tile) of every size

@@ -285,3 +285,21 @@ def test_walk_order_does_not_change_results(shape, prec):
         p.c2r(a)
         torch.cuda.synchronize()
         assert torch.equal(a[..., :nz], b[..., :nz])
+
+
+def test_misaligned_data_pointers_rejected_before_any_launch():
+    """Every kernel moves whole complex elements: a pointer off by one real
+    is a layout error and is rejected (GFFT_ERR_INVALID_ARG) with the array
+    untouched."""
+    import torch
+    for prec, dt, off in (("f32", torch.float32, 4), ("f64", torch.float64, 8)):
+        buf = torch.zeros((4, 8, 18), dtype=dt, device="cuda:0")
+        keep = buf.clone()
+        with G.Plan(4, 8, 16, precision=prec) as p:
+            bad = buf.data_ptr() + off
+            for call in (lambda: p.r2c(bad), lambda: p.c2r(bad), lambda: p.axis(1, False, bad)):
+                with pytest.raises(G.GfftError) as e:
+                    call()
+                assert e.value.status == 1
+        torch.cuda.synchronize()
+        assert torch.equal(buf, keep)

@@ -135,6 +135,9 @@ def test_over_budget_and_bad_requests_rejected():
         with pytest.raises(G.GfftError) as e:  # prime factor 17: no plan
             srv.submit((17, 17, 34), "f32", S.R2C, x)
         assert e.value.status == 3
+        with pytest.raises(G.GfftError) as e:  # not aligned to a complex element
+            srv.submit((8, 8, 8), "f32", S.R2C, x.data_ptr() + 4, device=True)
+        assert e.value.status == 1
         host = torch.zeros(8, 8, 10)
         with pytest.raises(G.GfftError) as e:
             srv.submit((8, 8, 8), "f32", S.R2C, host, device=True)  # not device memory
