@@ -624,6 +624,7 @@ int tune(PlanCore& c, void* scratch, hipStream_t s, int reps) {
     return GFFT_ERR_HIP;
   }
   int st = GFFT_OK;
+  std::vector<float> iso[3];  // isolated pair times per candidate, axes timed here
   for (int axis = 0; axis < 3 && st == GFFT_OK; ++axis) {
     AxisPlan* a = axis_ptr(c, axis);
     if (!a->active || a->cands.size() < 2) continue;
@@ -660,6 +661,7 @@ int tune(PlanCore& c, void* scratch, hipStream_t s, int reps) {
         break;
       }
       ms /= reps;
+      iso[axis].push_back(ms);
       if (ms < best_ms) {
         best_ms = ms;
         best = v;
@@ -670,6 +672,57 @@ int tune(PlanCore& c, void* scratch, hipStream_t s, int reps) {
     a->tuned_ms = best_ms;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = cand_desc(*a, best);
+  }
+  // In-context refinement: an axis timed alone misses how its walk meets the
+  // neighbouring passes' (the memory-side cache still holds the tiles a pass
+  // wrote last), so the three fastest candidates of each axis timed above
+  // are timed again inside whole R2C + C2R round trips.
+  if (st == GFFT_OK && c.procs == 1) {
+    const int fused_keep = c.fused;
+    c.fused = 0;  // the z and y choices matter only without the plane kernels
+    auto round_trip_ms = [&](float* ms) -> int {
+      int r = exec_r2c(c, scratch, s);
+      if (r == GFFT_OK) r = exec_c2r(c, scratch, s);
+      if (r != GFFT_OK) return r;
+      if (hipEventRecord(e0, s) != hipSuccess) return GFFT_ERR_HIP;
+      for (int k = 0; k < reps && r == GFFT_OK; ++k) {
+        r = exec_r2c(c, scratch, s);
+        if (r == GFFT_OK) r = exec_c2r(c, scratch, s);
+      }
+      if (r != GFFT_OK) return r;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+        return GFFT_ERR_HIP;
+      *ms /= reps;
+      return GFFT_OK;
+    };
+    for (int axis = 0; axis < 3 && st == GFFT_OK; ++axis) {
+      const std::vector<float>& t = iso[axis];
+      if (t.size() < 2) continue;
+      AxisPlan* a = axis_ptr(c, axis);
+      std::vector<int> order(t.size());
+      for (int v = 0; v < (int)t.size(); ++v) order[v] = v;
+      std::sort(order.begin(), order.end(), [&](int x, int y) { return t[x] < t[y]; });
+      int best = a->variant;
+      float best_rt = 0;
+      if ((st = round_trip_ms(&best_rt)) != GFFT_OK) break;
+      for (int k = 0; k < 3 && k < (int)order.size() && st == GFFT_OK; ++k) {
+        if (order[k] == best) continue;
+        set_variant(c, axis, order[k]);
+        float rt = 0;
+        if ((st = round_trip_ms(&rt)) != GFFT_OK) break;
+        if (rt < 0.997f * best_rt) {  // a margin against timing noise
+          best_rt = rt;
+          best = order[k];
+        }
+      }
+      if (st != GFFT_OK) break;
+      set_variant(c, axis, best);
+      a->tuned_ms = t[best];
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune_cache[tune_key(c, axis)] = cand_desc(*a, best);
+    }
+    c.fused = fused_keep;
   }
   // fused z + y plane kernels against the (tuned) separate passes
   if (st == GFFT_OK && c.plane && c.d_twz_plane) {
