@@ -18,7 +18,8 @@ AGREE = {"f32": 2e-7, "f64": 1e-15}
 # every compiled plane geometry, plus odd Nx and Nx = 1
 CASES = [((64, 64, 64), "f32"), ((96, 96, 96), "f32"), ((128, 128, 128), "f32"), ((160, 160, 160), "f32"),
          ((64, 64, 64), "f64"), ((96, 96, 96), "f64"), ((128, 128, 128), "f64"), ((45, 128, 128), "f64"),
-         ((1, 96, 96), "f32"), ((5, 64, 64), "f64"), ((27, 160, 160), "f32")]
+         ((192, 192, 192), "f32"), ((1, 96, 96), "f32"), ((5, 64, 64), "f64"), ((27, 160, 160), "f32"),
+         ((7, 192, 192), "f32")]
 
 
 def say(*a):
