@@ -19,12 +19,15 @@ int registry_plane(const PlaneEntry** out) {
       // column-scratch kernels (256-512 threads), whole round trips: FP32 64
       // -15 %, 96 -22 %, 128 -4 % (512 threads; 1024 was 4 % slower), 160
       // -8 %; FP64 64 -11 %, 96 -15 %, 128 -12 %.  FP32 192 fits only with the
-      // in-place exchange (150.5 KB): 78.9 -> 72.3 us against separate passes.
+      // in-place exchange (150.5 KB): 78.9 -> 72.3 us against separate passes;
+      // FP32 144 72.5 -> 68.9 us (in place; FP32 80 and FP64 80 in place were
+      // slower or level and are left out).
       entry_plane<float, Sc64, 4, Sc64, 4, 64, true>(),
       entry_plane<float, Sc96, 6, Sc96, 6, 64, true>(),
       entry_plane<float, Sc128, 8, Sc128, 4, 32, true>(),
       entry_plane<float, Sc160, 10, Sc160, 5, 32, true>(),
       entry_plane<float, Sc192, 24, Sc192, 24, 64, true>(),
+      entry_plane<float, Sc144, 36, Sc144, 36, 80, true>(),
       entry_plane<double, Sc64, 4, Sc64, 4, 64, true>(),
       entry_plane<double, Sc96, 6, Sc96, 6, 64, true>(),
       entry_plane<double, Sc128, 8, Sc128, 4, 32, true>(),

@@ -19,7 +19,7 @@ AGREE = {"f32": 2e-7, "f64": 1e-15}
 CASES = [((64, 64, 64), "f32"), ((96, 96, 96), "f32"), ((128, 128, 128), "f32"), ((160, 160, 160), "f32"),
          ((64, 64, 64), "f64"), ((96, 96, 96), "f64"), ((128, 128, 128), "f64"), ((45, 128, 128), "f64"),
          ((192, 192, 192), "f32"), ((1, 96, 96), "f32"), ((5, 64, 64), "f64"), ((27, 160, 160), "f32"),
-         ((7, 192, 192), "f32")]
+         ((7, 192, 192), "f32"), ((144, 144, 144), "f32"), ((9, 144, 144), "f32")]
 
 
 def say(*a):
