@@ -1,7 +1,7 @@
 """GPU parity of the in-place 3D R2C/C2R path against the FP64 oracle, the
 reference's recorded output and the reference plan geometry on hipFFT.
 
-Tolerances (north star): relative L2 <= 1e-6 for FP32, <= 1e-13 for FP64,
+Tolerances: relative L2 <= 1e-6 for FP32, <= 1e-13 for FP64,
 against the FP64 oracle on the same seeded inputs.  Every transform goes
 through libgpufft.so's C ABI.
 """
