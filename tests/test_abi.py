@@ -159,3 +159,21 @@ def test_tune_cache_text_round_trip_and_validation():
     assert G.lib().gfft_tune_cache_export(small, 4, None) == 9
     G.clear_tune_cache()
     assert G.tune_cache_text() == ""
+
+
+def test_header_compiles_as_c_and_links_from_c(tmp_path):
+    """include/gpufft/gpufft.h as plain C11 (gcc), linked against the library,
+    running host-only entry points."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "gpu_fft_amd")
+    exe = str(tmp_path / "abi_c_consumer")
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "c_abi", "abi_c_consumer.c"), "-L", libdir, "-lgpufft",
+                    "-Wl,-rpath," + libdir, "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "FAIL" not in out.stdout and out.stdout.count("ok ") == 9
