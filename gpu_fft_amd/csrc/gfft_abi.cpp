@@ -77,6 +77,7 @@ int gfft_plan_create(gfft_plan* plan, int64_t nx, int64_t ny, int64_t nz, int pr
     delete p;
     return st;
   }
+  (void)gfft::apply_tune_cache(p->core);  // tuned choices of this geometry, if any
   *plan = p;
   return GFFT_OK;
 }

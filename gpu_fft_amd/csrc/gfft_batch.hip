@@ -587,6 +587,7 @@ int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec)
         free_twiddles(*pc);
         break;
       }
+      (void)apply_tune_cache(*pc);
       it = b->plans.emplace(key, std::move(pc)).first;
     }
     b->xplan[j] = it->second.get();

@@ -566,6 +566,12 @@ std::string tune_key(const PlanCore& c, int axis) {
   return device_tag() + "|" + b;
 }
 
+std::string plane_key(const PlanCore& c) {
+  char b[128];
+  std::snprintf(b, sizeof b, "p%d:plane:%lld:%lld:%lld", c.prec, (long long)c.nx, (long long)c.ny, (long long)c.nz);
+  return device_tag() + "|" + b;
+}
+
 // description of candidate v of an axis: kernel shape and radix order
 std::string cand_desc(const AxisPlan& a, int v) {
   const KernelEntry* e = a.cands[v].first;
@@ -578,6 +584,37 @@ std::string cand_desc(const AxisPlan& a, int v) {
   return b;
 }
 }  // namespace
+
+int apply_tune_cache(PlanCore& c) {
+  std::string keys[3];
+  for (int axis = 0; axis < 3; ++axis) {
+    const AxisPlan* a = axis_ptr(c, axis);
+    if (a->active && a->cands.size() >= 2) keys[axis] = tune_key(c, axis);
+  }
+  const std::string kb = c.plane ? plane_key(c) : std::string();
+  int applied = 0;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int axis = 0; axis < 3; ++axis) {
+    if (keys[axis].empty()) continue;
+    auto it = g_tune_cache.find(keys[axis]);
+    if (it == g_tune_cache.end()) continue;
+    AxisPlan* a = axis_ptr(c, axis);
+    for (int v = 0; v < (int)a->cands.size(); ++v)
+      if (cand_desc(*a, v) == it->second) {
+        set_variant(c, axis, v);
+        ++applied;
+        break;
+      }
+  }
+  if (!kb.empty()) {
+    auto it = g_tune_cache.find(kb);
+    if (it != g_tune_cache.end() && (it->second == "fused:0" || it->second == "fused:1")) {
+      c.fused = it->second == "fused:1" ? 1 : 0;
+      ++applied;
+    }
+  }
+  return applied;
+}
 
 std::string tune_cache_text() {
   std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -726,10 +763,7 @@ int tune(PlanCore& c, void* scratch, hipStream_t s, int reps) {
   }
   // fused z + y plane kernels against the (tuned) separate passes
   if (st == GFFT_OK && c.plane && c.d_twz_plane) {
-    char kb0[128];
-    std::snprintf(kb0, sizeof kb0, "p%d:plane:%lld:%lld:%lld", c.prec, (long long)c.nx, (long long)c.ny,
-                  (long long)c.nz);
-    const std::string kb = device_tag() + "|" + kb0;
+    const std::string kb = plane_key(c);
     int cached = -1;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -72,6 +72,10 @@ int exec_r2c_finish(const PlanCore& c, const void* recvbuf, void* out, hipStream
 int exec_c2r_start(const PlanCore& c, const void* data, void* sendbuf, hipStream_t s);
 int exec_c2r_local(const PlanCore& c, const void* recvbuf, void* data, hipStream_t s);
 int tune(PlanCore& c, void* scratch, hipStream_t s, int reps);
+// Set the variants (and fused mode) the process-wide tune cache holds for
+// this plan's geometry on the current device; call after upload_twiddles.
+// Returns the number of choices applied.
+int apply_tune_cache(PlanCore& c);
 ColGeom y_geom(const PlanCore& c, double scale);
 ColGeom x_geom(const PlanCore& c, double scale);
 RowGeom z_geom(const PlanCore& c, double scale);

@@ -128,6 +128,7 @@ int get_plan(gfft_server_s* s, int64_t nx, int64_t ny, int64_t nz, int prec, con
     gfft::free_twiddles(*p);
     return st;
   }
+  (void)gfft::apply_tune_cache(*p);  // tuned choices of this geometry, if any
   s->plan_bytes += gfft::plan_device_bytes(*p);
   *out = p.get();
   s->plans.emplace(k, std::move(p));

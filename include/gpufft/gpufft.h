@@ -74,7 +74,11 @@ const char* gfft_strerror(int status);
  * picks a kernel per axis, uploads exact twiddle tables (computed in extended
  * precision on the host, rounded once).  Unlike the reference it allocates
  * no work buffer: every pass is in place.  procs > 1 plans the local share of
- * a slab decomposition (see gfft_exec_r2c_local). */
+ * a slab decomposition (see gfft_exec_r2c_local).  Where the process-wide
+ * tune cache holds choices for this geometry on this device type (an earlier
+ * gfft_plan_tune, or gfft_tune_cache_import), the plan starts with them;
+ * otherwise with the default variants.  Serving and batched plans do the
+ * same. */
 int gfft_plan_create(gfft_plan* plan, int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int prec);
 
 /* Release a plan and its device tables (the reference never frees). */

@@ -46,3 +46,26 @@ def test_exported_cache_reproduces_the_tuned_plan(tmp_path):
     got = G.complex_view(x, *shape).cpu().numpy()
     assert O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), shape[2])) <= 1e-6
     G.clear_tune_cache()
+
+
+def test_new_plans_start_with_the_cached_choices():
+    """A plan created after a tune of its geometry (or an imported cache)
+    starts with the tuned variants; with the cache cleared it starts with
+    the defaults.  512^3 FP32, whose tuned z and x kernels differ from the
+    defaults (persistent row tiles, straight-line column tiles)."""
+    import torch
+    n = 512
+    G.clear_tune_cache()
+    with G.Plan(n, n, n) as p:
+        fresh = {a: p.describe()[a]["variant"] for a in "zyx"}
+    assert fresh == {"z": 0, "y": 0, "x": 0}
+    buf = torch.zeros((n, n, 2 * (n // 2 + 1)), device="cuda:0")
+    with G.Plan(n, n, n) as p:
+        p.tune(buf, reps=2)
+        tuned = {a: p.describe()[a]["variant"] for a in "zyx"}
+    assert tuned != fresh
+    with G.Plan(n, n, n) as p:
+        assert {a: p.describe()[a]["variant"] for a in "zyx"} == tuned
+    G.clear_tune_cache()
+    with G.Plan(n, n, n) as p:
+        assert {a: p.describe()[a]["variant"] for a in "zyx"} == fresh
