@@ -635,7 +635,7 @@ def main():
                                           "gflops_batch", "roundtrip_max_rel_err", "variants")}
         out["batch"]["workload"] = b["config"]["workload"]
         log("serving: device-resident requests")
-        out["serving"] = serve_bench(1500, staged=False, load=0.8)
+        out["serving"] = serve_bench(5000, staged=False, load=0.8)
         log("serving: host-staged requests")
         out["serving_host_staged"] = serve_bench(300, staged=True, load=0.8)
     print(json.dumps(out), flush=True)
