@@ -27,7 +27,7 @@ struct FlatLds {
   __device__ __forceinline__ cx<T>& at(int n) { return b[n ^ ((n >> 3) & 7)]; }
 };
 
-// y exchange inside the plane's own column (IP plane kernels): entry n of
+// y exchange inside the plane's own column: entry n of
 // the column starting at b, s entries apart
 template <typename T>
 struct StrideLds {
@@ -36,14 +36,13 @@ struct StrideLds {
   __device__ __forceinline__ cx<T>& at(int n) { return b[n * s]; }
 };
 
-// IP = false: the y exchange runs through a column scratch of Ny x CW
-// entries.  IP = true: it runs in the columns themselves (the engine's
-// barrier before its first exchange write follows the column loads, and one
-// more barrier precedes the output writes), so only a tail of max(Ny, K)
-// entries follows the plane: lanes past the last column exchange there, and
-// an odd Ny's last z row runs its exchange into it.  Without the scratch a
-// workgroup can hold more columns (threads) or a CU more workgroups.
-template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW, bool IP = false>
+// The y exchange runs in the columns themselves (the engine's barrier before
+// its first exchange write follows the column loads, and one more barrier
+// precedes the output writes), so no Ny x CW column scratch is needed: only a
+// tail of max(Ny, K) entries follows the plane.  Lanes past the last column
+// exchange there, and an odd Ny's last z row runs its exchange into it.  The
+// LDS a scratch would take goes to wider column groups (more threads).
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW>
 struct PlanePass {
   using C = cx<T>;
   static constexpr int NZ = SCZ::N, NY = SCY::N, K = NZ / 2 + 1;
@@ -56,7 +55,7 @@ struct PlanePass {
   static constexpr int GZ = NT / TPTZ;  // row pairs in flight
   static constexpr int NPAIRS = (NY + 1) / 2;
   static constexpr int PLANE = NY * K;  // complex entries of one x-plane
-  static constexpr int SCRATCH = IP ? (NY > K ? NY : K) : NY * CW;
+  static constexpr int SCRATCH = NY > K ? NY : K;  // the exchange tail
   static constexpr size_t LDS_BYTES = (size_t)(PLANE + SCRATCH) * sizeof(C);
   static_assert(64 % TPTZ == 0, "the z exchange must stay inside one wavefront");
   static_assert(NT % TPTZ == 0 && NT <= 1024, "thread count");
@@ -198,14 +197,9 @@ struct PlanePass {
         for (int r = 0; r < E::R0; ++r)
           v[q * E::R0 + r] = valid ? plane[E::in_index(t, q, r) * K + k] : mk<T>(0, 0);
       }
-      if constexpr (IP) {
-        StrideLds<T> L{valid ? plane + k : scratch, valid ? K : 1};
-        E::run(v, t, L, tw);
-        __syncthreads();  // every exchange read of the columns is done
-      } else {
-        ColLds<T, CW> L{scratch, c};
-        E::run(v, t, L, tw);
-      }
+      StrideLds<T> L{valid ? plane + k : scratch, valid ? K : 1};
+      E::run(v, t, L, tw);
+      __syncthreads();  // every exchange read of the columns is done
       if (valid) {
 #pragma unroll
         for (int q = 0; q < E::NBL; ++q) {
@@ -222,10 +216,10 @@ struct PlanePass {
   }
 };
 
-template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW, bool IP>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, PlanePass<T, SCZ, PZ, SCY, PY, CW, IP>::NT))) void
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, PlanePass<T, SCZ, PZ, SCY, PY, CW>::NT))) void
 k_plane_r2c(T* data, const cx<T>* twz, const cx<T>* twy, double yscale) {
-  using K_ = PlanePass<T, SCZ, PZ, SCY, PY, CW, IP>;
+  using K_ = PlanePass<T, SCZ, PZ, SCY, PY, CW>;
   __shared__ cx<T> smem[K_::PLANE + K_::SCRATCH];
   cx<T>* g = reinterpret_cast<cx<T>*>(data) + (size_t)blockIdx.x * K_::PLANE;
   K_::copy_in(smem, g);
@@ -237,10 +231,10 @@ k_plane_r2c(T* data, const cx<T>* twz, const cx<T>* twy, double yscale) {
   for (int i = threadIdx.x; i < K_::PLANE; i += K_::NT) st_stream(g + i, smem[i]);
 }
 
-template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW, bool IP>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, PlanePass<T, SCZ, PZ, SCY, PY, CW, IP>::NT))) void
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, PlanePass<T, SCZ, PZ, SCY, PY, CW>::NT))) void
 k_plane_c2r(T* data, const cx<T>* twz, const cx<T>* twy, double zscale) {
-  using K_ = PlanePass<T, SCZ, PZ, SCY, PY, CW, IP>;
+  using K_ = PlanePass<T, SCZ, PZ, SCY, PY, CW>;
   __shared__ cx<T> smem[K_::PLANE + K_::SCRATCH];
   cx<T>* g = reinterpret_cast<cx<T>*>(data) + (size_t)blockIdx.x * K_::PLANE;
   K_::copy_in(smem, g);
@@ -259,32 +253,32 @@ k_plane_c2r(T* data, const cx<T>* twz, const cx<T>* twy, double zscale) {
   }
 }
 
-template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW, bool IP, bool INV>
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW, bool INV>
 hipError_t launch_plane(void* data, int64_t nplanes, double scale, const void* twz, const void* twy,
                         hipStream_t s) {
-  using K_ = PlanePass<T, SCZ, PZ, SCY, PY, CW, IP>;
+  using K_ = PlanePass<T, SCZ, PZ, SCY, PY, CW>;
   if (nplanes <= 0) return hipSuccess;
   if (INV)
-    hipLaunchKernelGGL((k_plane_c2r<T, SCZ, PZ, SCY, PY, CW, IP>), dim3((uint32_t)nplanes), dim3(K_::NT), 0, s,
+    hipLaunchKernelGGL((k_plane_c2r<T, SCZ, PZ, SCY, PY, CW>), dim3((uint32_t)nplanes), dim3(K_::NT), 0, s,
                        static_cast<T*>(data), static_cast<const cx<T>*>(twz), static_cast<const cx<T>*>(twy),
                        scale);
   else
-    hipLaunchKernelGGL((k_plane_r2c<T, SCZ, PZ, SCY, PY, CW, IP>), dim3((uint32_t)nplanes), dim3(K_::NT), 0, s,
+    hipLaunchKernelGGL((k_plane_r2c<T, SCZ, PZ, SCY, PY, CW>), dim3((uint32_t)nplanes), dim3(K_::NT), 0, s,
                        static_cast<T*>(data), static_cast<const cx<T>*>(twz), static_cast<const cx<T>*>(twy),
                        scale);
   return hipGetLastError();
 }
 
-template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW, bool IP = false>
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW>
 PlaneEntry entry_plane() {
   return PlaneEntry{prec_of<T>(),
                     SCZ::N,
                     SCY::N,
                     make_sched<SCZ>(),
                     make_sched<SCY>(),
-                    PlanePass<T, SCZ, PZ, SCY, PY, CW, IP>::NT,
-                    reinterpret_cast<void*>(&launch_plane<T, SCZ, PZ, SCY, PY, CW, IP, false>),
-                    reinterpret_cast<void*>(&launch_plane<T, SCZ, PZ, SCY, PY, CW, IP, true>)};
+                    PlanePass<T, SCZ, PZ, SCY, PY, CW>::NT,
+                    reinterpret_cast<void*>(&launch_plane<T, SCZ, PZ, SCY, PY, CW, false>),
+                    reinterpret_cast<void*>(&launch_plane<T, SCZ, PZ, SCY, PY, CW, true>)};
 }
 
 }  // namespace gfft
