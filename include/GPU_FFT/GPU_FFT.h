@@ -25,15 +25,35 @@
 //   * MPI is optional: <mpi.h> is used when present (or when
 //     GPU_FFT_WITH_MPI is defined), otherwise MPI_Comm is a placeholder
 //     type and the communicator argument is ignored, as it is at p = 1.
+//     INIT_GPU_FFT_COMM<T1,T2> keeps the reference's bookkeeping (rank,
+//     procs, communicator; communications_setup.cu:56-71) and
+//     __get_mpi_datatype__ maps T2_f/T2_d to MPI_C_COMPLEX /
+//     MPI_C_DOUBLE_COMPLEX (communications_setup.cu:43-53) when <mpi.h> is
+//     in use.
 #ifndef GPU_FFT_DROPIN_H_
 #define GPU_FFT_DROPIN_H_
 
+// The reference defines this macro in its public headers (GPU_FFT.h:40-46,
+// transitions.h:39-45) and its callers use it (GPU_FFT_TEST_SCRIPT/test.cu:41).
+#ifndef inline_qualifier
+#if defined(_MSC_VER)
+#define inline_qualifier __inline
+#else
+#define inline_qualifier inline
+#endif
+#endif
+
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <initializer_list>
+#include <iostream>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "gpufft/gpufft.h"
 
@@ -109,6 +129,36 @@ inline int __procs__ = 1;
 inline int __rank__ = 0;
 inline MPI_Comm __MPI_COMMUNICATOR__{};
 
+#if GPU_FFT_HAVE_MPI
+// Request slots for the 2(p-1) point-to-point transfers of one exchange
+// (communications_setup.cu:67). Owned here (the reference leaks them); this
+// library posts no transfers itself, so they stay unused.
+inline std::vector<MPI_Request> __requests__;
+
+// MPI type of one complex element (communications_setup.cu:43-53).
+template <typename T>
+MPI_Datatype __get_mpi_datatype__(T a);
+template <>
+inline MPI_Datatype __get_mpi_datatype__(float2) { return MPI_C_COMPLEX; }
+template <>
+inline MPI_Datatype __get_mpi_datatype__(double2) { return MPI_C_DOUBLE_COMPLEX; }
+#endif
+
+// Communication bookkeeping, as the reference's INIT_GPU_FFT_COMM
+// (communications_setup.cu:56-71; declared at GPU_FFT.h:111-112): records
+// the communicator, rank and rank count. INIT_GPU_FFT calls it, as the
+// reference's INIT does (GPU_FFT_init.cu:57).
+template <typename T1, typename T2>
+void INIT_GPU_FFT_COMM(int procs, int rank, MPI_Comm& MPI_COMMUNICATOR_INPUT) {
+  if (procs < 1 || rank < 0 || rank >= procs) throw Error(GFFT_ERR_INVALID_ARG, "INIT_GPU_FFT_COMM");
+  __MPI_COMMUNICATOR__ = MPI_COMMUNICATOR_INPUT;
+  __rank__ = rank;
+  __procs__ = procs;
+#if GPU_FFT_HAVE_MPI
+  __requests__.assign(2 * static_cast<size_t>(procs - 1), MPI_REQUEST_NULL);
+#endif
+}
+
 namespace detail {
 
 template <typename T1, typename T2>
@@ -141,9 +191,7 @@ void INIT_GPU_FFT(int64 Nx_in, int64 Ny_in, int64 Nz_in, int procs, int rank, MP
   __Nx__ = Nx_in;
   __Ny__ = Ny_in;
   __Nz__ = Nz_in;
-  __procs__ = procs;
-  __rank__ = rank;
-  __MPI_COMMUNICATOR__ = MPI_COMMUNICATOR_INPUT;
+  INIT_GPU_FFT_COMM<T1, T2>(procs, rank, MPI_COMMUNICATOR_INPUT);
 }
 
 template <typename T1, typename T2>

@@ -177,3 +177,41 @@ def test_header_compiles_as_c_and_links_from_c(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "FAIL" not in out.stdout and out.stdout.count("ok ") == 9
+
+
+REF_TEST_CU = "/root/reference/GPU_FFT_TEST_SCRIPT/test.cu"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TEST_CU), reason="reference sources not present")
+def test_reference_caller_compiles_and_links_unchanged(tmp_path):
+    """The reference's own caller (GPU_FFT_TEST_SCRIPT/test.cu, with its
+    header.h) compiled in place, unmodified, against include/GPU_FFT/GPU_FFT.h
+    and linked against libgpufft.so and MPI. Nothing is copied; the object and
+    executable go to a temporary directory."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    mpi_inc = next((d for d in ("/opt/conda/include", "/usr/include/mpich", "/usr/include/openmpi")
+                    if os.path.exists(os.path.join(d, "mpi.h"))), None)
+    if mpi_inc is None:
+        pytest.skip("no mpi.h")
+    obj = str(tmp_path / "test.o")
+    r = subprocess.run([hipcc, "-std=c++17", "-w", "--offload-arch=gfx950", "-c", REF_TEST_CU,
+                        "-I", os.path.join(ROOT, "include"), "-I", mpi_inc, "-o", obj],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    mpi_lib = os.path.join(os.path.dirname(mpi_inc), "lib")
+    if not os.path.exists(os.path.join(mpi_lib, "libmpi.so")):
+        return
+    libdir = os.path.join(ROOT, "gpu_fft_amd")
+    exe = str(tmp_path / "GPU_FFT_test")
+    # libmpi by full path: a -L on MPI's lib directory would also pick up any
+    # older libstdc++ installed beside it
+    libmpi = os.path.join(mpi_lib, "libmpi.so")
+    r = subprocess.run([hipcc, "--offload-arch=gfx950", obj, "-L", libdir, "-lgpufft", "-Wl,-rpath," + libdir,
+                        libmpi, "-Wl,-rpath-link," + mpi_lib, "-o", exe],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert os.path.getsize(exe) > 0
