@@ -58,16 +58,19 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(shape, reps: int, workers: int) -> dict:
-    """The FP64 oracle (pocketfft via scipy.fft, multithreaded) on the same
-    transform: R2C + C2R of the seeded field, median of ``reps`` runs."""
+def cpu_baseline(shape, prec: str, reps: int, workers: int) -> dict:
+    """The oracle's p = 1 transform (pocketfft via scipy.fft, multithreaded)
+    in the GPU line's precision: R2C + C2R of the seeded field, median of
+    ``reps`` runs.  FFTW3 is not installed on these machines."""
+    import numpy as np
     from oracle import gfft_oracle as O
     nx, ny, nz = shape
-    f = O.pad_real(O.random_field(nx, ny, nz, seed=0))
+    dt = np.float32 if prec == "f32" else np.float64
+    f = O.pad_real(O.random_field(nx, ny, nz, seed=0)).astype(dt)
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        s = O.r2c_p1(f, nz, workers=workers)
+        s = O.r2c_p1(f, nz, workers=workers, dtype=dt)
         O.c2r_p1(s, nz, workers=workers)
         times.append(time.perf_counter() - t0)
         del s
@@ -79,8 +82,9 @@ def cpu_baseline(shape, reps: int, workers: int) -> dict:
         "unit": "GFLOP/s",
         "cores": workers,
         "kind": "port",
-        "sample": (f"FP64 oracle (scipy.fft pocketfft, {workers} threads) R2C+C2R of {nx}x{ny}x{nz}, median of {reps} "
-                   f"runs ({t * 1e3:.1f} ms each), CPU {cpu_model()}, {os.cpu_count()} logical CPUs"),
+        "sample": (f"pocketfft (scipy), {workers} threads -- not FFTW3: {prec.upper()} R2C+C2R of "
+                   f"{nx}x{ny}x{nz} (the oracle's p=1 transform in the GPU line's precision), median of {reps} "
+                   f"runs ({t * 1e3:.1f} ms each); CPU {cpu_model()}, {os.cpu_count()} logical CPUs"),
     }
 
 
@@ -97,49 +101,91 @@ def measured_traffic(pass_name: str, n: int, prec: str):
     return int(v["read_bytes"] + v["write_bytes"]) if v else None
 
 
-def slab_share(args, nx, ny, nz, procs):
+def measure_slab_share(nx, ny, nz, procs, prec, steps, warmup, tune_reps=3) -> dict:
     """One rank's share of a slab decomposition: the local R2C half (z R2C +
     y C2C with the slab-transpose pack fused into the y stores), on the
-    (Nx/p, Ny, 2K) block.  Algorithmic bytes: the unfused model's 6 S_local
-    (z, y, pack each read + write), the fused data movement is 4 S_local."""
+    (Nx/p, Ny, 2K) block.  Data moved: 4 S_local (z read + write, y read +
+    write into the p destination chunks); the unfused model (z, y, pack
+    each read + write) would move 6 S_local."""
     import torch
     import gpu_fft_amd as G
     K = nz // 2 + 1
     nxl = nx // procs
-    rb = 4 if args.prec == "f32" else 8
+    rb = 4 if prec == "f32" else 8
     S = nxl * ny * K * 2 * rb
     dev = torch.device("cuda:0")
-    dt = torch.float32 if args.prec == "f32" else torch.float64
+    dt = torch.float32 if prec == "f32" else torch.float64
     gen = torch.Generator(device=dev)
     gen.manual_seed(0)
     data = torch.rand((nxl, ny, 2 * K), dtype=dt, device=dev, generator=gen) * 2 - 1
     send = torch.empty_like(data)
-    plan = G.Plan(nx, ny, nz, procs=procs, rank=0, precision=args.prec)
-    if not args.no_tune:
-        plan.tune(send, reps=3)
+    plan = G.Plan(nx, ny, nz, procs=procs, rank=0, precision=prec)
+    if tune_reps > 0:
+        plan.tune(send, reps=tune_reps)
     stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         plan.r2c_local(data, send, stream)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    for _ in range(args.steps):
+    for _ in range(steps):
         plan.r2c_local(data, send, stream)
     e1.record(stream)
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
+    ms = e0.elapsed_time(e1) / steps
+    variants = {ax: plan.describe()[ax] for ax in ("z", "y")}
+    plan.close()
+    del data, send
+    torch.cuda.empty_cache()
+    return {"ms": ms, "S_local": S, "variants": variants,
+            "data_moved_4S_GBps": round(4 * S / (ms * 1e-3) / 1e9, 1),
+            "frac_4S": round(4 * S / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "frac_unfused_6S_model": round(6 * S / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+
+
+def slab_share(args, nx, ny, nz, procs):
+    m = measure_slab_share(nx, ny, nz, procs, args.prec, args.steps, args.warmup, 0 if args.no_tune else 3)
+    ms = m["ms"]
     out = {"metric": "p-share local R2C half (z R2C + y C2C + pack) time", "value": round(ms, 5), "unit": "ms",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
            "higher_is_better": False, "dtype": args.prec,
-           "config": {"workload": f"rank share of {nx}x{ny}x{nz} at p={procs}: {nxl}x{ny}x{nz} local block",
-                      "S_local": S},
-           "roofline": {"bound": "hbm", "alg_bytes_unfused_6S": 6 * S,
-                        "frac_unfused_model": round(6 * S / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
-                        "data_moved_4S_GBps": round(4 * S / (ms * 1e-3) / 1e9, 1),
-                        "frac_fused_data": round(4 * S / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
-           "variants": {ax: plan.describe()[ax] for ax in ("z", "y")}}
-    plan.close()
+           "config": {"workload": f"rank share of {nx}x{ny}x{nz} at p={procs}: {nx // procs}x{ny}x{nz} local block",
+                      "S_local": m["S_local"]},
+           "roofline": {"bound": "hbm", "data_moved_4S_GBps": m["data_moved_4S_GBps"], "frac_4S": m["frac_4S"],
+                        "frac_unfused_6S_model": m["frac_unfused_6S_model"]},
+           "variants": m["variants"]}
     print(json.dumps(out), flush=True)
+
+
+# Config grids measured beside the headline in the default line (every one
+# tuned outside its timed region; 10 timed round trips each).
+CONFIG_GRIDS = [((128, 128, 128), "f64"), ((512, 512, 512), "f64"), ((1024, 1024, 1024), "f32"),
+                ((1024, 512, 2048), "f32"), ((768, 768, 768), "f32")]
+
+
+def config_results(steps: int = 10, warmup: int = 2) -> dict:
+    """ms per round trip, the 12 S model fraction and the dominant pass of
+    every config grid, plus the p = 8 share's local R2C half of 1024^3."""
+    out = {}
+    for (nx, ny, nz), prec in CONFIG_GRIDS:
+        log(f"config {nx}x{ny}x{nz} {prec}")
+        m = measure_round_trip(nx, ny, nz, prec, steps, warmup, tune_reps=3)
+        S, ms = m["S"], m["ms_step"]
+        dom = m["dom"]
+        out[f"{nx}x{ny}x{nz}_{prec}"] = {
+            "ms_per_round_trip": round(ms, 5),
+            "frac_12S": round(12 * S / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "dominant_pass": dom,
+            "dominant_GBps": m["per_pass"][dom]["GBps"],
+            "per_pass_GBps": {k: v["GBps"] for k, v in m["per_pass"].items()},
+        }
+    for prec in ("f32", "f64"):
+        log(f"config p=8 share of 1024^3 {prec}")
+        m = measure_slab_share(1024, 1024, 1024, 8, prec, 20, 3, tune_reps=3)
+        out[f"p8_share_1024cube_local_r2c_{prec}"] = {
+            "ms": round(m["ms"], 5), "frac_4S": m["frac_4S"], "data_moved_4S_GBps": m["data_moved_4S_GBps"],
+            "model": "4 S_local moved (z read+write, y read+write into the 8 chunks)"}
+    return out
 
 
 BATCH_SIZES = [32, 48, 64, 96, 128, 160, 192, 256]
@@ -260,7 +306,8 @@ def pct(xs, q):
 
 
 def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: float = 64.0, nstreams: int = 4,
-                seed: int = 0, small_lane: int = -1, own_queues: bool = True) -> dict:
+                seed: int = 0, small_lane: int = -1, own_queues: bool = True, stream_depth: int = -1,
+                poll_us: int = -1) -> dict:
     """Poisson-arrival serving run of the config-5 mix.
 
     1. Plans of all 84 (size, precision) geometries are built up front
@@ -286,6 +333,10 @@ def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: flo
     srv = SV.Server(int(budget_gb * (1 << 30)), nstreams=nstreams, staging=staged, own_queues=own_queues)
     if small_lane >= 0:
         srv.set_small_lane(small_lane)
+    if stream_depth >= 0:
+        srv.set_stream_depth(stream_depth)
+    if poll_us >= 0:
+        srv.set_poll_interval(poll_us)
     for n, p in keys:
         srv.prepare(n, n, n, p)
     bufs, hin, hout = {}, {}, {}
@@ -342,6 +393,8 @@ def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: flo
     ok = [r for r in res if r["status"] == 0]
     lat = [(r["done"] - r["arrival"]) * 1e3 for r in ok]
     queue = [(r["admitted"] - r["arrival"]) * 1e3 for r in ok]
+    squeue = [(r["started"] - r["admitted"]) * 1e3 for r in ok]
+    service = [(r["done"] - r["started"]) * 1e3 for r in ok]
     span = max(r["done"] for r in ok) - float(sched[0])
     bytes_total = sum(r["bytes"] for r in ok)
     flops = sum(5.0 * n ** 3 * math.log2(n ** 3) for n, _, _ in reqs)
@@ -365,11 +418,17 @@ def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: flo
         "latency_ms_by_edge": {k: {"n": len(v), "p50": round(pct(v, 50), 4), "p99": round(pct(v, 99), 4)}
                                for k, v in by_edge.items() if v},
         "queue_ms": {"p50": round(pct(queue, 50), 4), "p99": round(pct(queue, 99), 4)},
+        "stream_queue_ms": {"p50": round(pct(squeue, 50), 4), "p99": round(pct(squeue, 99), 4)},
+        "service_ms": {"p50": round(pct(service, 50), 4), "p99": round(pct(service, 99), 4)},
+        "latency_parts": "queue = admitted - arrival (server FIFO), stream_queue = device start - admitted "
+                         "(behind earlier requests on its stream), service = done - device start",
         "gpu_ms_mean": round(sum(r["gpu_ms"] for r in ok) / len(ok), 4),
         "staged_ms_mean": round(sum(r["staged_ms"] for r in ok) / len(ok), 4),
         "budget_bytes": st["budget"], "peak_in_flight_bytes": st["peak_in_flight_bytes"],
         "plans_resident": st["plans"], "plan_bytes": st["plan_bytes"], "streams": nstreams,
         "small_lane_bytes": small_lane if small_lane >= 0 else (32 << 20 if nstreams >= 2 else 0),
+        "stream_depth": stream_depth if stream_depth >= 0 else 1,
+        "waiter_poll_us": poll_us if poll_us >= 0 else 20,
         "max_submit_lateness_ms": round(late * 1e3, 4),
         "workload": "edges log-uniform in [64,512] -> nearest of 42 sizes 2^a3^b5^c, 80/20 FP32/FP64, "
                     "R2C/C2R 50/50, Poisson arrivals, seed 0",
@@ -377,9 +436,13 @@ def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: flo
 
 
 def serve_main(args) -> dict:
-    dev_res = serve_bench(args.serve, staged=False, load=args.serve_load, small_lane=args.serve_small_lane)
+    dev_res = serve_bench(args.serve, staged=False, load=args.serve_load, small_lane=args.serve_small_lane,
+                          stream_depth=args.serve_depth, poll_us=args.serve_poll_us, nstreams=args.serve_streams)
+    if args.serve_device_only:
+        return {"device": dev_res}
     host_res = serve_bench(max(100, args.serve // 5), staged=True, load=args.serve_load,
-                           small_lane=args.serve_small_lane)
+                           small_lane=args.serve_small_lane, stream_depth=args.serve_depth,
+                           poll_us=args.serve_poll_us, nstreams=args.serve_streams)
     return {"metric": "served mixed-size R2C/C2R requests: p50/p99 arrival-to-completion latency at "
                       f"{args.serve_load:.0%} of capacity",
             "value": dev_res["latency_ms"]["p99"], "unit": "ms (p99 latency, device-resident requests)",
@@ -431,6 +494,122 @@ def accuracy(cases) -> dict:
     return out
 
 
+def copy_calibration(nx, ny, nz, prec, steps):
+    """A plain device copy of S bytes, for HBM counter calibration under rocprofv3."""
+    import torch
+    K = nz // 2 + 1
+    dt = torch.float32 if prec == "f32" else torch.float64
+    buf = torch.rand((nx, ny, 2 * K), dtype=dt, device="cuda:0")
+    dst = torch.empty_like(buf)
+    for _ in range(steps):
+        dst.copy_(buf)
+    torch.cuda.synchronize()
+    S = buf.numel() * buf.element_size()
+    print(json.dumps({"copy_calibration_bytes_read": S, "copy_calibration_bytes_written": S,
+                      "launches": steps}), flush=True)
+
+
+def measure_round_trip(nx, ny, nz, prec, steps, warmup, tune_reps=10):
+    """R2C + C2R round trips of the seeded uniform field on one plan (tuned
+    outside the timed region when tune_reps > 0).  Two regions of `steps`
+    steps: the timed one (gfft_exec_r2c + gfft_exec_c2r per step, HIP events
+    around the region) and the per-pass one (one event pair around each
+    pass)."""
+    import torch
+    import gpu_fft_amd as G
+    K = nz // 2 + 1
+    real_bytes = 4 if prec == "f32" else 8
+    S = nx * ny * K * 2 * real_bytes
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    dt = torch.float32 if prec == "f32" else torch.float64
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    buf = torch.rand((nx, ny, 2 * K), dtype=dt, device=dev, generator=gen) * 2 - 1
+    buf[:, :, nz:] = 0
+    stream = torch.cuda.current_stream()
+
+    plan = G.Plan(nx, ny, nz, precision=prec)
+    plan.set_scale(1.0, 1.0 / (nx * ny * nz))
+    if tune_reps > 0:
+        # plan-time autotuning (outside the timed region): it overwrites the
+        # buffer, which is regenerated from the same seed afterwards
+        plan.tune(buf, reps=tune_reps)
+        torch.cuda.synchronize()
+        gen.manual_seed(0)
+        buf.copy_(torch.rand((nx, ny, 2 * K), dtype=dt, device=dev, generator=gen) * 2 - 1)
+        buf[:, :, nz:] = 0
+    desc = plan.describe()
+    log("plan", json.dumps(desc))
+
+    if desc.get("zy_fused"):
+        # small planes: z and y run as one fused LDS-resident plane pass
+        # (axis 3); its HBM bytes per launch are 2 S like every other pass
+        passes = [(3, False), (2, False), (2, True), (3, True)]
+        names = ["zy_r2c", "x_fwd", "x_inv", "zy_c2r"]
+    else:
+        passes = [(0, False), (1, False), (2, False), (2, True), (1, True), (0, True)]
+        names = ["z_r2c", "y_fwd", "x_fwd", "x_inv", "y_inv", "z_c2r"]
+
+    def step():
+        plan.r2c(buf, stream)
+        plan.c2r(buf, stream)
+
+    def step_passes(ev):
+        for j, (ax, inv) in enumerate(passes):
+            ev[j][0].record(stream)
+            plan.axis(ax, inv, buf, stream)
+            ev[j][1].record(stream)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region: exactly K steps through the R2C / C2R entry points
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    t_start.record(stream)
+    for k in range(steps):
+        step()
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms_total = t_start.elapsed_time(t_end)
+    ms_step = ms_total / steps
+
+    # per-pass region: K more steps, one event pair around each pass
+    evs = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)] for _ in passes]
+           for _ in range(steps)]
+    p_start = torch.cuda.Event(enable_timing=True)
+    p_end = torch.cuda.Event(enable_timing=True)
+    p_start.record(stream)
+    for k in range(steps):
+        step_passes(evs[k])
+    p_end.record(stream)
+    torch.cuda.synchronize()
+    ms_step_passes = p_start.elapsed_time(p_end) / steps
+
+    per_pass = {}
+    for j, nm in enumerate(names):
+        ts = [evs[k][j][0].elapsed_time(evs[k][j][1]) for k in range(steps)]
+        mean = sum(ts) / len(ts)
+        per_pass[nm] = {"ms_mean": round(mean, 5), "ms_median": round(statistics.median(ts), 5),
+                        "ms_min": round(min(ts), 5), "GBps": round(2 * S / (mean * 1e-3) / 1e9, 1)}
+    # sanity: the array must still hold the (normalised) input's magnitude
+    rms = float(buf[:, :, :nz].float().pow(2).mean().sqrt())
+    finite = bool(torch.isfinite(buf[:, :, :nz]).all())
+    log(f"ms/step {ms_step:.4f}  wall/step {wall / steps * 1e3:.4f}  rms {rms:.4f} finite {finite}")
+
+    plan.close()
+    del buf
+    torch.cuda.empty_cache()
+    dom = max(per_pass, key=lambda k: per_pass[k]["ms_mean"])
+    return {"S": S, "ms_step": ms_step, "ms_step_passes": ms_step_passes, "per_pass": per_pass, "names": names,
+            "dom": dom, "desc": desc, "wall_s": wall, "rms": rms, "finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -447,6 +626,12 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.8, help="offered load as a fraction of capacity")
     ap.add_argument("--serve-small-lane", type=int, default=-1, metavar="BYTES",
                     help="reserved small-request stream threshold (0 = off, -1 = server default)")
+    ap.add_argument("--serve-depth", type=int, default=-1, metavar="D",
+                    help="unfinished requests per server stream (0 = no cap, -1 = server default 1)")
+    ap.add_argument("--serve-poll-us", type=int, default=-1, metavar="US",
+                    help="server waiters' sleep between completion queries (0 = blocking, -1 = default 20)")
+    ap.add_argument("--serve-device-only", action="store_true", help="serving: device requests only")
+    ap.add_argument("--serve-streams", type=int, default=4, help="server streams")
     ap.add_argument("--accuracy", action="store_true",
                     help="relative L2 error vs the FP64 oracle at every config grid (no timing)")
     ap.add_argument("--no-extras", action="store_true",
@@ -482,105 +667,15 @@ def main():
         print(json.dumps({"metric": "relative L2 error vs the FP64 oracle per config",
                           "results": accuracy(ACCURACY_CASES)}), flush=True)
         return
-    K = nz // 2 + 1
-    real_bytes = 4 if args.prec == "f32" else 8
-    S = nx * ny * K * 2 * real_bytes
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    dt = torch.float32 if args.prec == "f32" else torch.float64
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(0)
-    buf = torch.rand((nx, ny, 2 * K), dtype=dt, device=dev, generator=gen) * 2 - 1
-    buf[:, :, nz:] = 0
-    stream = torch.cuda.current_stream()
-
     if args.copy_calibration:
-        dst = torch.empty_like(buf)
-        for _ in range(args.steps):
-            dst.copy_(buf)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_read": S, "copy_calibration_bytes_written": S,
-                          "launches": args.steps}), flush=True)
+        copy_calibration(nx, ny, nz, args.prec, args.steps)
         return
-
-    plan = G.Plan(nx, ny, nz, precision=args.prec)
-    plan.set_scale(1.0, 1.0 / (nx * ny * nz))
-    if not args.no_tune:
-        # plan-time autotuning (outside the timed region): it overwrites the
-        # buffer, which is regenerated from the same seed afterwards
-        plan.tune(buf, reps=10)
-        torch.cuda.synchronize()
-        gen.manual_seed(0)
-        buf.copy_(torch.rand((nx, ny, 2 * K), dtype=dt, device=dev, generator=gen) * 2 - 1)
-        buf[:, :, nz:] = 0
-    desc = plan.describe()
-    log("plan", json.dumps(desc))
-
-    if desc.get("zy_fused"):
-        # small planes: z and y run as one fused LDS-resident plane pass
-        # (axis 3); its HBM bytes per launch are 2 S like every other pass
-        passes = [(3, False), (2, False), (2, True), (3, True)]
-        names = ["zy_r2c", "x_fwd", "x_inv", "zy_c2r"]
-    else:
-        passes = [(0, False), (1, False), (2, False), (2, True), (1, True), (0, True)]
-        names = ["z_r2c", "y_fwd", "x_fwd", "x_inv", "y_inv", "z_c2r"]
-
-    def step():
-        plan.r2c(buf, stream)
-        plan.c2r(buf, stream)
-
-    def step_passes(ev):
-        for j, (ax, inv) in enumerate(passes):
-            ev[j][0].record(stream)
-            plan.axis(ax, inv, buf, stream)
-            ev[j][1].record(stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    # timed region: exactly K steps through the R2C / C2R entry points
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    w0 = time.perf_counter()
-    t_start.record(stream)
-    for k in range(args.steps):
-        step()
-    t_end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    ms_total = t_start.elapsed_time(t_end)
-    ms_step = ms_total / args.steps
-
-    # per-pass region: K more steps, one event pair around each pass
-    evs = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)] for _ in passes]
-           for _ in range(args.steps)]
-    p_start = torch.cuda.Event(enable_timing=True)
-    p_end = torch.cuda.Event(enable_timing=True)
-    p_start.record(stream)
-    for k in range(args.steps):
-        step_passes(evs[k])
-    p_end.record(stream)
-    torch.cuda.synchronize()
-    ms_step_passes = p_start.elapsed_time(p_end) / args.steps
-
-    per_pass = {}
-    for j, nm in enumerate(names):
-        ts = [evs[k][j][0].elapsed_time(evs[k][j][1]) for k in range(args.steps)]
-        mean = sum(ts) / len(ts)
-        per_pass[nm] = {"ms_mean": round(mean, 5), "ms_median": round(statistics.median(ts), 5),
-                        "ms_min": round(min(ts), 5), "GBps": round(2 * S / (mean * 1e-3) / 1e9, 1)}
-    dom = max(per_pass, key=lambda k: per_pass[k]["ms_mean"])
-    dom_ms = per_pass[dom]["ms_mean"]
-    achieved = 2 * S / (dom_ms * 1e-3) / 1e9
+    m = measure_round_trip(nx, ny, nz, args.prec, args.steps, args.warmup, tune_reps=0 if args.no_tune else 10)
+    S, ms_step, per_pass, desc = m["S"], m["ms_step"], m["per_pass"], m["desc"]
+    names, dom = m["names"], m["dom"]
+    achieved = 2 * S / (per_pass[dom]["ms_mean"] * 1e-3) / 1e9
     traffic = measured_traffic(dom, n, args.prec) if (nx, ny, nz) == (n, n, n) else None
-
-    # sanity: the array must still hold the (normalised) input's magnitude
-    rms = float(buf[:, :, :nz].float().pow(2).mean().sqrt())
-    finite = bool(torch.isfinite(buf[:, :, :nz]).all())
-    log(f"ms/step {ms_step:.4f}  wall/step {wall / args.steps * 1e3:.4f}  rms {rms:.4f} finite {finite}")
-
+    ms_step_passes, wall, rms, finite = m["ms_step_passes"], m["wall_s"], m["rms"], m["finite"]
     flops_step = 2 * 5.0 * (nx * ny * nz) * math.log2(nx * ny * nz)
     gflops = flops_step / (ms_step * 1e-3) / 1e9
     total_alg = 12 * S
@@ -619,16 +714,14 @@ def main():
         "output_rms": round(rms, 5),
         "output_finite": finite,
     }
-    plan.close()
-    del buf
-    torch.cuda.empty_cache()
     if not args.no_cpu:
         workers = max(1, min(16, os.cpu_count() or 1))
         log(f"cpu baseline: {workers} threads")
-        out["cpu_baseline"] = cpu_baseline((nx, ny, nz), args.cpu_reps, workers)
+        out["cpu_baseline"] = cpu_baseline((nx, ny, nz), args.prec, args.cpu_reps, workers)
     else:
         out["cpu_baseline"] = None
     if not args.no_extras:
+        out["configs"] = config_results()
         log("batched mode: 256 mixed cubes")
         b = batch_bench(argparse.Namespace(batch=256, steps=20, warmup=3))
         out["batch"] = {k: b[k] for k in ("plain_ms", "batch_graph_ms", "batch_eager_ms", "speedup_vs_plain",

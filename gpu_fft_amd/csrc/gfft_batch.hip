@@ -286,7 +286,14 @@ uint32_t tiles_of(const BXform& x, int axis, int w) {
   return (nc + w - 1) / w;
 }
 
+// An executable may still run on any stream the caller launched it on (or
+// be queued behind other work there), so the whole device is synchronised
+// before any executable or graph is destroyed.
 void drop_graphs(gfft_batch_s* b) {
+  bool any = false;
+  for (int m = 0; m < 3; ++m) any = any || b->gexec[m] || b->graph[m];
+  if (!any) return;
+  (void)hipDeviceSynchronize();
   for (int m = 0; m < 3; ++m) {
     if (b->gexec[m]) (void)hipGraphExecDestroy(b->gexec[m]);
     if (b->graph[m]) (void)hipGraphDestroy(b->graph[m]);

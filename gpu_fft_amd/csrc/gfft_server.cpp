@@ -1,12 +1,17 @@
 // Serving runtime: requests of mixed sizes / precisions / directions are
 // admitted in arrival order under an HBM budget and run on a few internal
-// streams; a poller thread stamps completions and admits the next requests.
+// streams; one waiter thread per stream blocks on the completion event of
+// that stream's oldest request, stamps the completion and admits the next
+// requests.
 //
 // A request's life (host-staged):
-//   submit -> waiting FIFO -> admitted (budget + arena block) ->
-//   H2D copy, transform passes, D2H copy on one internal stream ->
-//   poller sees the last event -> block freed, result queued.
-// Device requests skip the copies and are transformed in place.
+//   submit -> waiting FIFO (small or large) -> admitted (budget, a stream
+//   with room, arena block) -> H2D copy, transform passes, D2H copy on that
+//   stream -> its waiter returns from the last event -> block freed, result
+//   queued.
+// Device requests skip the copies and are transformed in place; with a
+// producer stream named, the server's stream first waits for an event
+// recorded on the producer at submission.
 //
 // Plans are built once per (Nx, Ny, Nz, prec) and shared by every request of
 // that geometry; exec only reads a plan, so the same plan runs on several
@@ -32,8 +37,8 @@
 #include "gpufft/gpufft.h"
 
 // The ABI structs are bound field by field from other languages.
-static_assert(sizeof(gfft_request) == 80, "gfft_request layout");
-static_assert(sizeof(gfft_result) == 72, "gfft_result layout");
+static_assert(sizeof(gfft_request) == 88, "gfft_request layout");
+static_assert(sizeof(gfft_result) == 80, "gfft_result layout");
 static_assert(sizeof(gfft_server_stats_t) == 96, "gfft_server_stats_t layout");
 
 namespace {
@@ -43,11 +48,24 @@ using Clock = std::chrono::steady_clock;
 // default bound of the reserved stream's requests: 32 MiB (a 200^3 FP32 or a
 // 160^3 FP64 array)
 constexpr size_t kDefaultSmallBytes = size_t(32) << 20;
+// default cap of admitted, unfinished requests per stream: one, so a
+// request is bound to a stream only when that stream is free and every wait
+// happens (and is stamped) in the server's FIFO.  Measured on MI355X (5000
+// served requests, 80 % load): capacity and p99 equal to depth 2 or no cap
+// within run-to-run noise.
+constexpr size_t kDefaultDepth = 1;
+// default sleep between completion queries of a stream's waiter, in
+// microseconds (0: block in hipEventSynchronize; its interrupt-driven
+// wake-up was measured ~1.5 ms late on MI355X, too slow to keep the
+// depth-capped streams fed)
+constexpr int kDefaultPollUs = 20;
 
 double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
 
 struct Events {
-  hipEvent_t e[4] = {};  // before copy-in, before transform, after transform, after copy-out
+  // before copy-in, before transform, after transform, after copy-out,
+  // producer's work done (recorded on the producer stream at submission)
+  hipEvent_t e[5] = {};
 };
 
 struct Req {
@@ -57,6 +75,8 @@ struct Req {
   bool device = false;
   const gfft::PlanCore* plan = nullptr;
   size_t off = gfft::Arena::kNone;
+  int queue = 0;   // admission FIFO: 1 = at or under the small-lane threshold
+  bool after_producer = false;
   int stream = -1;
   int ev = -1;     // index into the event pool
   Events e;        // copy of the pool entry's handles (the pool may grow meanwhile)
@@ -64,11 +84,13 @@ struct Req {
   hipEvent_t done_event() const { return e.e[device ? 2 : 3]; }
 };
 
-// Requests on one stream finish in submission order: only the oldest one
-// needs to be checked.
+// Requests on one stream finish in submission order: the stream's waiter
+// only ever waits for the oldest one.
 struct Lane {
   std::deque<Req*> q;
   size_t bytes = 0;  // array bytes of the lane's unfinished requests
+  std::condition_variable cv;  // waiter: a request was queued, or stop
+  std::thread waiter;
 };
 
 using Key = std::tuple<int64_t, int64_t, int64_t, int>;
@@ -91,11 +113,13 @@ struct gfft_server_s {
   gfft::Arena arena;
   gfft::Admission<Req*> adm;
   std::vector<hipStream_t> streams;
-  std::vector<Lane> lanes;  // one per stream
+  std::vector<std::unique_ptr<Lane>> lanes;  // one per stream
   size_t n_running = 0;
   // requests of at most this many bytes may use the last stream, which no
   // larger request uses (0: no reserved stream)
   size_t small_bytes = 0;
+  size_t max_depth = kDefaultDepth;  // unfinished requests per stream (0: no cap)
+  int poll_us = kDefaultPollUs;      // waiter: sleep between queries (0: blocking synchronise)
   std::map<Key, std::unique_ptr<gfft::PlanCore>> plans;
   std::map<Key, int> checked;  // host-side plan status of geometries seen at submission
   size_t plan_bytes = 0;
@@ -105,9 +129,7 @@ struct gfft_server_s {
   size_t done_head = 0;
   uint64_t next_id = 1, submitted = 0, completed = 0, rejected = 0;
   std::mutex mu;
-  std::condition_variable cv_work;  // poller: something is running
-  std::condition_variable cv_idle;  // waiters: nothing waiting or running
-  std::thread poller;
+  std::condition_variable cv_idle;  // gfft_server_wait: nothing waiting or running
   bool stop = false;
 };
 
@@ -142,8 +164,9 @@ int take_events(gfft_server_s* s) {
     return i;
   }
   Events e;
-  for (int j = 0; j < 4; ++j)
-    if (hipEventCreate(&e.e[j]) != hipSuccess) {
+  // blocking events: a waiter sleeps in hipEventSynchronize instead of spinning
+  for (int j = 0; j < 5; ++j)
+    if (hipEventCreateWithFlags(&e.e[j], hipEventBlockingSync) != hipSuccess) {
       for (int q = 0; q < j; ++q) (void)hipEventDestroy(e.e[q]);
       return -1;
     }
@@ -161,7 +184,8 @@ void finish(gfft_server_s* s, Req* q, int status, double t_done) {
   r.admitted = q->t_adm;
   r.done = t_done;
   r.bytes = q->bytes;
-  if (q->ev >= 0 && status == GFFT_OK) {
+  r.started = t_done;
+  if (q->t_adm > 0 && q->ev >= 0 && status == GFFT_OK) {
     const Events& e = q->e;
     float a = 0, b = 0;
     if (hipEventElapsedTime(&a, e.e[1], e.e[2]) == hipSuccess) r.gpu_ms = a;
@@ -169,10 +193,13 @@ void finish(gfft_server_s* s, Req* q, int status, double t_done) {
       r.staged_ms = r.gpu_ms;
     else if (hipEventElapsedTime(&b, e.e[0], e.e[3]) == hipSuccess)
       r.staged_ms = b;
+    // the device reached the request's first event staged_ms before its last
+    r.started = t_done - r.staged_ms * 1e-3;
+    if (r.started < q->t_adm) r.started = q->t_adm;
   }
   if (q->ev >= 0) s->ev_free.push_back(q->ev);
   if (q->off != gfft::Arena::kNone) s->arena.release(q->off, q->bytes);
-  if (q->stream >= 0) s->lanes[q->stream].bytes -= q->bytes;
+  if (q->stream >= 0) s->lanes[q->stream]->bytes -= q->bytes;
   if (q->t_adm > 0) s->adm.finished(q->bytes);
   s->done.push_back(r);
   ++s->completed;
@@ -184,6 +211,7 @@ int launch(gfft_server_s* s, Req* q) {
   const hipStream_t st = s->streams[q->stream];
   const Events& e = q->e;
   void* data = q->device ? q->r.data : s->arena_base + q->off;
+  if (q->after_producer && hipStreamWaitEvent(st, e.e[4], 0) != hipSuccess) return GFFT_ERR_HIP;
   if (!q->device) {
     if (hipEventRecord(e.e[0], st) != hipSuccess) return GFFT_ERR_HIP;
     if (hipMemcpyAsync(data, q->r.data, q->bytes, hipMemcpyHostToDevice, st) != hipSuccess) return GFFT_ERR_HIP;
@@ -208,108 +236,109 @@ int launch(gfft_server_s* s, Req* q) {
   return GFFT_OK;
 }
 
-// Admit from the head of the FIFO while the head fits.  Caller holds mu.
+// Admit FIFO heads while one fits: of the two FIFOs' heads the earlier
+// arrival is tried first; a head that cannot start yet (budget, no stream
+// with room, no arena block) does not block the other FIFO.  Caller holds mu.
 void admit(gfft_server_s* s) {
-  bool started = false;
-  while (!s->adm.empty()) {
-    Req* q = s->adm.head();
-    if (!s->adm.fits(q->bytes)) break;
-    size_t off = gfft::Arena::kNone;
-    if (!q->device) {
-      off = s->arena.alloc(q->bytes);
-      if (off == gfft::Arena::kNone) break;
-    }
-    s->adm.pop_admitted(q->bytes);
-    q->off = off;
-    q->t_adm = now_s();
-    int st = q->plan ? GFFT_OK : get_plan(s, q->r.nx, q->r.ny, q->r.nz, q->r.prec, &q->plan);
-    if (st == GFFT_OK) {
-      q->ev = take_events(s);
-      if (q->ev < 0)
-        st = GFFT_ERR_HIP;
-      else
-        q->e = s->ev_pool[q->ev];
-    }
-    if (st == GFFT_OK) {
-      // the stream with the least outstanding work (array bytes); with a
-      // reserved stream, larger requests never queue on it, so small ones
-      // always have a stream without a large request ahead of them
-      const int nl = (int)s->lanes.size();
-      const bool reserve = s->small_bytes > 0 && nl >= 2;
-      const bool small = reserve && q->bytes <= s->small_bytes;
-      int best = -1;
+  const int nl = (int)s->lanes.size();
+  const bool reserve = s->small_bytes > 0 && nl >= 2;
+  std::vector<size_t> depth(nl), lbytes(nl);
+  for (;;) {
+    int order[2] = {0, 1};
+    if (!s->adm.empty(0) && !s->adm.empty(1) && s->adm.head(1)->id < s->adm.head(0)->id) std::swap(order[0], order[1]);
+    bool admitted = false;
+    for (int k = 0; k < 2 && !admitted; ++k) {
+      const int fq = order[k];
+      if (s->adm.empty(fq)) continue;
+      Req* q = s->adm.head(fq);
+      if (!s->adm.fits(q->bytes)) continue;
       for (int i = 0; i < nl; ++i) {
-        if (reserve && !small && i == nl - 1) continue;
-        if (best < 0 || s->lanes[i].bytes < s->lanes[best].bytes) best = i;
+        depth[i] = s->lanes[i]->q.size();
+        lbytes[i] = s->lanes[i]->bytes;
       }
-      q->stream = best;
-      s->lanes[best].bytes += q->bytes;
-      st = launch(s, q);
+      const int lane = gfft::pick_stream(depth.data(), lbytes.data(), nl, reserve, fq == 1, s->max_depth);
+      if (lane < 0) continue;
+      size_t off = gfft::Arena::kNone;
+      if (!q->device) {
+        off = s->arena.alloc(q->bytes);
+        if (off == gfft::Arena::kNone) continue;
+      }
+      s->adm.pop_admitted(q->bytes, fq);
+      admitted = true;
+      q->off = off;
+      q->t_adm = now_s();
+      int st = q->plan ? GFFT_OK : get_plan(s, q->r.nx, q->r.ny, q->r.nz, q->r.prec, &q->plan);
+      if (st == GFFT_OK) {
+        q->stream = lane;
+        s->lanes[lane]->bytes += q->bytes;
+        st = launch(s, q);
+      }
+      if (st != GFFT_OK) {
+        // nothing of it may still be in flight when its block is reused
+        if (q->stream >= 0) (void)hipStreamSynchronize(s->streams[q->stream]);
+        finish(s, q, st, now_s());
+        continue;
+      }
+      s->lanes[lane]->q.push_back(q);
+      ++s->n_running;
+      s->lanes[lane]->cv.notify_one();
     }
-    if (st != GFFT_OK) {
-      // nothing of it may still be in flight when its block is reused
-      if (q->stream >= 0) (void)hipStreamSynchronize(s->streams[q->stream]);
-      finish(s, q, st, now_s());
-      continue;
-    }
-    s->lanes[q->stream].q.push_back(q);
-    ++s->n_running;
-    started = true;
+    if (!admitted) break;
   }
-  if (started) s->cv_work.notify_one();
   if (s->adm.empty() && s->n_running == 0) s->cv_idle.notify_all();
 }
 
-// The lock is not held while events are queried: only this thread removes
-// requests from the lanes, so the oldest request of a lane stays valid.
-void poll_loop(gfft_server_s* s) {
+// Waiter of one stream: waits for the completion of the stream's oldest
+// request with the server lock released, either sleeping poll_us between
+// event queries or blocking in hipEventSynchronize (blocking-sync events);
+// only this thread removes requests from its lane, so the oldest request
+// stays valid.
+void lane_loop(gfft_server_s* s, int li) {
   (void)hipSetDevice(s->device);
-  std::vector<std::pair<int, Req*>> fronts;
+  Lane& L = *s->lanes[li];
   for (;;) {
+    Req* q = nullptr;
+    int poll_us = 0;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_work.wait(lk, [&] { return s->stop || s->n_running > 0; });
-      if (s->stop) return;
-      fronts.clear();
-      for (int i = 0; i < (int)s->lanes.size(); ++i)
-        if (!s->lanes[i].q.empty()) fronts.emplace_back(i, s->lanes[i].q.front());
+      L.cv.wait(lk, [&] { return s->stop || !L.q.empty(); });
+      if (L.q.empty()) return;  // stop requested and nothing left
+      q = L.q.front();
+      poll_us = s->poll_us;
     }
-    bool any = false;
-    for (const auto& f : fronts) {
-      const hipError_t e = hipEventQuery(f.second->done_event());
-      if (e == hipErrorNotReady) continue;
-      const double t = now_s();
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->lanes[f.first].q.pop_front();
-      --s->n_running;
-      finish(s, f.second, e == hipSuccess ? GFFT_OK : GFFT_ERR_HIP, t);
-      any = true;
-    }
-    if (any) {
-      std::lock_guard<std::mutex> lk(s->mu);
-      admit(s);
+    hipError_t e;
+    if (poll_us <= 0) {
+      e = hipEventSynchronize(q->done_event());
     } else {
-      std::this_thread::yield();
+      while ((e = hipEventQuery(q->done_event())) == hipErrorNotReady)
+        std::this_thread::sleep_for(std::chrono::microseconds(poll_us));
     }
+    const double t = now_s();
+    std::lock_guard<std::mutex> lk(s->mu);
+    L.q.pop_front();
+    --s->n_running;
+    finish(s, q, e == hipSuccess ? GFFT_OK : GFFT_ERR_HIP, t);
+    admit(s);
   }
 }
 
 void teardown(gfft_server_s* s) {
-  if (s->poller.joinable()) {
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->stop = true;
-    }
-    s->cv_work.notify_all();
-    s->poller.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  for (auto& l : s->lanes) {
+    l->cv.notify_all();
+    if (l->waiter.joinable()) l->waiter.join();
   }
   for (hipStream_t st : s->streams) (void)hipStreamSynchronize(st);
-  for (Lane& l : s->lanes)
-    for (Req* q : l.q) delete q;
-  while (!s->adm.empty()) {
-    delete s->adm.head();
-    s->adm.pop_admitted(0);
-  }
+  for (auto& l : s->lanes)
+    for (Req* q : l->q) delete q;
+  for (int fq = 0; fq < 2; ++fq)
+    while (!s->adm.empty(fq)) {
+      delete s->adm.head(fq);
+      s->adm.pop_admitted(0, fq);
+    }
   for (auto& kv : s->plans) gfft::free_twiddles(*kv.second);
   for (Events& e : s->ev_pool)
     for (hipEvent_t x : e.e)
@@ -333,6 +362,21 @@ int gfft_server_set_small_lane(gfft_server s, size_t max_bytes) {
   if (!s) return GFFT_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> lk(s->mu);
   s->small_bytes = max_bytes;
+  return GFFT_OK;
+}
+
+int gfft_server_set_poll_interval(gfft_server s, int microseconds) {
+  if (!s || microseconds < 0) return GFFT_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->poll_us = microseconds;
+  return GFFT_OK;
+}
+
+int gfft_server_set_stream_depth(gfft_server s, int depth) {
+  if (!s || depth < 0) return GFFT_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->max_depth = (size_t)depth;
+  admit(s);  // a larger cap may let waiting requests start now
   return GFFT_OK;
 }
 
@@ -381,11 +425,15 @@ int gfft_server_create(gfft_server* out, size_t budget_bytes, int nstreams, int 
       break;
     }
     s->streams.push_back(h);
-    s->lanes.emplace_back();
+    s->lanes.emplace_back(new (std::nothrow) Lane);
+    if (!s->lanes.back()) {
+      s->lanes.pop_back();
+      st = GFFT_ERR_OUT_OF_MEMORY;
+    }
   }
-  if (st == GFFT_OK) {
+  for (int i = 0; i < (int)s->lanes.size() && st == GFFT_OK; ++i) {
     try {
-      s->poller = std::thread(poll_loop, s);
+      s->lanes[i]->waiter = std::thread(lane_loop, s, i);
     } catch (...) {
       st = GFFT_ERR_OUT_OF_MEMORY;
     }
@@ -410,8 +458,8 @@ int gfft_server_prepare(gfft_server s, int64_t nx, int64_t ny, int64_t nz, int p
 }
 
 int gfft_server_submit(gfft_server s, const gfft_request* req, uint64_t* id) {
-  if (!s || !req || !req->data || (req->direction != 0 && req->direction != 1) || (req->flags & ~GFFT_REQ_DEVICE) ||
-      req->reserved != 0)
+  if (!s || !req || !req->data || (req->direction != 0 && req->direction != 1) ||
+      (req->flags & ~(GFFT_REQ_DEVICE | GFFT_REQ_PRODUCER)) || req->reserved != 0)
     return GFFT_ERR_INVALID_ARG;
   const bool device = (req->flags & GFFT_REQ_DEVICE) != 0;
   if (!device && (!req->host_out || !s->arena_base)) return GFFT_ERR_INVALID_ARG;
@@ -433,7 +481,10 @@ int gfft_server_submit(gfft_server s, const gfft_request* req, uint64_t* id) {
     }
     if (it->second != GFFT_OK) return it->second;
   }
-  if (bytes > s->adm.budget()) {
+  // a host-staged request needs one arena block (256-byte multiples, arena
+  // = the budget rounded down): one that can never exist would block its
+  // FIFO for good
+  if (bytes > s->adm.budget() || (!device && !s->arena.can_hold(bytes))) {
     std::lock_guard<std::mutex> lk(s->mu);
     ++s->rejected;
     return GFFT_ERR_OVER_BUDGET;
@@ -454,12 +505,32 @@ int gfft_server_submit(gfft_server s, const gfft_request* req, uint64_t* id) {
   q->r = *req;
   q->bytes = bytes;
   q->device = device;
+  // The legacy null stream as producer needs no event with the default
+  // (blocking) server streams: work queued on a blocking stream already
+  // waits for the null stream's earlier work.  An event recorded on the
+  // null stream would also wait for every blocking stream's queued work,
+  // serialising the server's streams.
+  q->after_producer = (req->flags & GFFT_REQ_PRODUCER) != 0 &&
+                      !(req->producer == nullptr && !(s->flags & GFFT_SERVER_SHARED_QUEUES));
   if (q->r.arrival < 0) q->r.arrival = now_s();
   std::lock_guard<std::mutex> lk(s->mu);
+  q->ev = take_events(s);
+  if (q->ev < 0) {
+    delete q;
+    return GFFT_ERR_HIP;
+  }
+  q->e = s->ev_pool[q->ev];
+  // the producer's work queued so far; the server's stream waits for it
+  if (q->after_producer && hipEventRecord(q->e.e[4], reinterpret_cast<hipStream_t>(req->producer)) != hipSuccess) {
+    s->ev_free.push_back(q->ev);
+    delete q;
+    return GFFT_ERR_HIP;
+  }
+  q->queue = (s->small_bytes > 0 && s->lanes.size() >= 2 && bytes <= s->small_bytes) ? 1 : 0;
   q->id = s->next_id++;
   ++s->submitted;
   if (id) *id = q->id;
-  s->adm.push(q);
+  s->adm.push(q, q->queue);
   admit(s);
   return GFFT_OK;
 }

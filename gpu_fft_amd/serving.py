@@ -23,6 +23,7 @@ R2C, C2R = 0, 1
 STAGING = 1
 SHARED_QUEUES = 2
 REQ_DEVICE = 1
+REQ_PRODUCER = 2
 
 def _is_235(n: int) -> bool:
     for p in (2, 3, 5):
@@ -39,14 +40,15 @@ class Request(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
                 ("prec", ctypes.c_int), ("direction", ctypes.c_int), ("flags", ctypes.c_int),
                 ("reserved", ctypes.c_int), ("data", ctypes.c_void_p), ("host_out", ctypes.c_void_p),
-                ("scale", ctypes.c_double), ("arrival", ctypes.c_double), ("tag", ctypes.c_uint64)]
+                ("scale", ctypes.c_double), ("arrival", ctypes.c_double), ("tag", ctypes.c_uint64),
+                ("producer", ctypes.c_void_p)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("tag", ctypes.c_uint64), ("status", ctypes.c_int),
                 ("stream", ctypes.c_int), ("arrival", ctypes.c_double), ("admitted", ctypes.c_double),
                 ("done", ctypes.c_double), ("gpu_ms", ctypes.c_double), ("staged_ms", ctypes.c_double),
-                ("bytes", ctypes.c_size_t)]
+                ("bytes", ctypes.c_size_t), ("started", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -73,6 +75,10 @@ def _bind(L):
     L.gfft_server_create.argtypes = [ctypes.POINTER(vp), sz, ctypes.c_int, ctypes.c_int]
     L.gfft_server_set_small_lane.restype = ctypes.c_int
     L.gfft_server_set_small_lane.argtypes = [vp, sz]
+    L.gfft_server_set_poll_interval.restype = ctypes.c_int
+    L.gfft_server_set_poll_interval.argtypes = [vp, ctypes.c_int]
+    L.gfft_server_set_stream_depth.restype = ctypes.c_int
+    L.gfft_server_set_stream_depth.argtypes = [vp, ctypes.c_int]
     L.gfft_server_prepare.restype = ctypes.c_int
     L.gfft_server_prepare.argtypes = [vp, i64, i64, i64, ctypes.c_int]
     L.gfft_server_submit.restype = ctypes.c_int
@@ -126,19 +132,44 @@ class Server:
         """Reserve the last stream for requests of at most max_bytes (0 = off)."""
         _check(lib().gfft_server_set_small_lane(self.handle, int(max_bytes)), "gfft_server_set_small_lane")
 
+    def set_stream_depth(self, depth: int) -> None:
+        """At most ``depth`` admitted, unfinished requests per stream (0 = no cap)."""
+        _check(lib().gfft_server_set_stream_depth(self.handle, int(depth)), "gfft_server_set_stream_depth")
+
+    def set_poll_interval(self, microseconds: int) -> None:
+        """Sleep between completion queries of the stream waiters (0 = block
+        in hipEventSynchronize)."""
+        _check(lib().gfft_server_set_poll_interval(self.handle, int(microseconds)), "gfft_server_set_poll_interval")
+
     def prepare(self, nx, ny, nz, precision="f32") -> None:
         _check(lib().gfft_server_prepare(self.handle, nx, ny, nz, _prec(precision)), "gfft_server_prepare")
 
     def submit(self, dims, precision, direction, data, host_out=None, device=None, scale=0.0, arrival=-1.0,
-               tag=0) -> int:
+               tag=0, producer="current") -> int:
         """Queue one request; returns its id.  ``device`` defaults to
-        ``data.is_cuda`` for tensors."""
+        ``data.is_cuda`` for tensors.  A device request starts after the
+        work queued so far on ``producer``: a torch stream, a raw stream
+        handle, ``"current"`` (default: torch's current stream when ``data``
+        is a CUDA tensor) or None (the caller guarantees ``data`` is ready)."""
         if device is None:
             device = bool(getattr(data, "is_cuda", False))
+        flags = REQ_DEVICE if device else 0
+        handle = None
+        if device and producer is not None:
+            if isinstance(producer, str):
+                if producer != "current":
+                    raise ValueError("producer must be 'current', a stream, a handle or None")
+                if getattr(data, "is_cuda", False):
+                    import torch
+                    handle = torch.cuda.current_stream(data.device).cuda_stream
+            else:
+                handle = int(getattr(producer, "cuda_stream", producer))
+        if handle is not None:
+            flags |= REQ_PRODUCER
         r = Request(int(dims[0]), int(dims[1]), int(dims[2]), _prec(precision), int(direction),
-                    REQ_DEVICE if device else 0, 0, _ptr(data),
+                    flags, 0, _ptr(data),
                     0 if device else _ptr(host_out if host_out is not None else data),
-                    float(scale), float(arrival), int(tag))
+                    float(scale), float(arrival), int(tag), handle or 0)
         rid = ctypes.c_uint64()
         _check(lib().gfft_server_submit(self.handle, ctypes.byref(r), ctypes.byref(rid)), "gfft_server_submit")
         if not isinstance(data, int):
@@ -191,4 +222,4 @@ class Server:
             pass
 
 
-__all__ += ["F32", "F64", "GfftError", "STAGING", "SHARED_QUEUES", "REQ_DEVICE"]
+__all__ += ["F32", "F64", "GfftError", "STAGING", "SHARED_QUEUES", "REQ_DEVICE", "REQ_PRODUCER"]

@@ -262,17 +262,22 @@ int gfft_batch_destroy(gfft_batch batch);
  * Serving runtime (no counterpart in the reference): R2C / C2R requests of
  * mixed sizes and precisions arrive one by one and are executed as they are
  * admitted.  Plans and twiddle tables stay resident per (Nx, Ny, Nz, prec).
- * Admission is strict arrival order under an HBM byte budget: the head
- * request starts once its array fits beside everything in flight (and, if
- * host-staged, once the server's arena has a free block for it).  Admitted
- * requests are spread over `nstreams` internal streams; a poller thread
- * stamps completions.  Every entry point is thread-safe.
+ * Admission is in arrival order (one FIFO for requests at or under the
+ * small-lane threshold, one for larger ones) under an HBM byte budget: a
+ * FIFO's head starts once its array fits beside everything in flight, a
+ * stream it may use has fewer than the stream depth of unfinished requests,
+ * and, if host-staged, the server's arena has a free block for it.  Admitted
+ * requests are spread over `nstreams` internal streams; one waiter thread
+ * per stream waits for its oldest request's completion event (timed polling
+ * or a blocking synchronise) and stamps the completion.  Every entry point
+ * is thread-safe.
  * ------------------------------------------------------------------------- */
 typedef struct gfft_server_s* gfft_server;
 
 #define GFFT_SERVER_STAGING 1 /* create: allocate a budget-sized arena for host-staged requests */
 #define GFFT_SERVER_SHARED_QUEUES 2 /* create: plain non-blocking streams from the runtime's queue pool */
 #define GFFT_REQ_DEVICE 1     /* request: data is device memory, transformed in place */
+#define GFFT_REQ_PRODUCER 2   /* request: start only after the work queued on `producer` before submit */
 
 typedef struct {
   int64_t nx, ny, nz;
@@ -285,6 +290,10 @@ typedef struct {
   double scale;   /* output scale fused into the last pass; 0 means 1 (unnormalised) */
   double arrival; /* arrival time on gfft_server_clock(); < 0 = at submission */
   uint64_t tag;   /* returned unchanged in the result */
+  gfft_stream_t producer; /* with GFFT_REQ_PRODUCER: the stream that writes `data` (an event is recorded
+                           * on it at submission and the server's stream waits for that event; for the
+                           * null stream with the default blocking server streams the ordering is
+                           * implicit and no event is recorded) */
 } gfft_request;
 
 typedef struct {
@@ -295,6 +304,8 @@ typedef struct {
   double gpu_ms;                  /* transform kernels only (device events) */
   double staged_ms;               /* copy in + transform + copy out (device events); = gpu_ms for device */
   size_t bytes;                   /* array bytes counted against the budget */
+  double started;                 /* when the device began the request's first operation:
+                                   * done - staged_ms (started - admitted = time in the stream's queue) */
 } gfft_result;
 
 typedef struct {
@@ -321,8 +332,20 @@ int gfft_server_create(gfft_server* srv, size_t budget_bytes, int nstreams, int 
  * larger requests never queue there, so a small request is never stuck
  * behind a large one on every stream.  Default with nstreams >= 2: 32 MiB;
  * 0 turns the reservation off (every request goes to the stream with the
- * least outstanding bytes).  Applies to requests admitted afterwards. */
+ * least outstanding bytes).  Applies to requests submitted afterwards. */
 int gfft_server_set_small_lane(gfft_server srv, size_t max_bytes);
+
+/* At most `depth` admitted, unfinished requests per stream (default 1: a
+ * request is bound to a stream once that stream is free); further requests
+ * wait in the server's FIFO, where their time shows as admitted - arrival.
+ * 0 = no limit.  Applies to admissions from now on. */
+int gfft_server_set_stream_depth(gfft_server srv, int depth);
+
+/* How a stream's waiter waits for its oldest request: sleeping
+ * `microseconds` between event queries (default 20), or with 0 blocking in
+ * hipEventSynchronize (no polling; wake-ups come later, so the streams may
+ * idle between requests). */
+int gfft_server_set_poll_interval(gfft_server srv, int microseconds);
 
 /* Build and keep the plan of a geometry now (otherwise built on first use,
  * which allocates and synchronises). */
@@ -330,7 +353,13 @@ int gfft_server_prepare(gfft_server srv, int64_t nx, int64_t ny, int64_t nz, int
 
 /* Queue a request.  Invalid geometry, sizes without a plan
  * (GFFT_ERR_UNSUPPORTED_SIZE), GFFT_ERR_OVER_BUDGET (the array alone exceeds
- * the budget) and bad pointers are rejected here, nothing queued. */
+ * the budget, or for a host-staged request the arena) and bad pointers are
+ * rejected here, nothing queued.  A device array must be complete when the
+ * server's stream reaches it: either its producer's work is finished before
+ * submit, or the request names the producer stream (GFFT_REQ_PRODUCER).
+ * Requests at or under the small-lane threshold and larger ones are admitted
+ * from two FIFOs, each in arrival order, so a small request never waits
+ * behind a large one. */
 int gfft_server_submit(gfft_server srv, const gfft_request* req, uint64_t* id);
 
 /* Non-blocking: move up to max finished results (completion order) to out. */

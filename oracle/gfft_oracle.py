@@ -295,12 +295,14 @@ def c2r(spec: np.ndarray, nz: int, workers: int = 1) -> np.ndarray:
     return gpu_fft_c2r_ranks([spec], nx, ny, nz, workers)[0]
 
 
-def r2c_p1(f_padded: np.ndarray, nz: int, workers: int = 1) -> np.ndarray:
+def r2c_p1(f_padded: np.ndarray, nz: int, workers: int = 1, dtype=np.float64) -> np.ndarray:
     """p = 1 forward without the identity permutations: at p = 1
     transpose_slab, the self-copy and chunk_transpose compose to the identity
     (pinned by test_multirank_gather_is_rfftn with procs = 1), so the
-    pipeline is rfftn of the unpadded field.  For large grids."""
-    real = np.asarray(f_padded, dtype=np.float64)[..., :nz]
+    pipeline is rfftn of the unpadded field.  For large grids.  dtype
+    float32 runs pocketfft in single precision (the CPU baseline of an FP32
+    GPU run); the checker always uses float64."""
+    real = np.asarray(f_padded, dtype=dtype)[..., :nz]
     if _sfft is not None:
         return _sfft.rfftn(real, workers=workers)
     return np.fft.rfftn(real)

@@ -96,7 +96,56 @@ static void admission_fifo() {
   CHECK(q.in_flight() == 0);
 }
 
+// Two FIFOs share one budget: the small queue's head is admitted while the
+// large queue's head still waits; each queue stays in its own arrival order.
+static void admission_two_queues() {
+  gfft::Admission<int> q(100);
+  q.push(1, 0);
+  q.push(2, 0);
+  q.push(3, 1);
+  q.push(4, 1);
+  CHECK(q.waiting() == 4);
+  q.pop_admitted(70, 0);           // 1 -> 70
+  CHECK(q.head(0) == 2 && !q.fits(50));
+  CHECK(q.head(1) == 3 && q.fits(20));
+  q.pop_admitted(20, 1);           // 3 runs beside 1 while 2 waits
+  CHECK(q.head(1) == 4 && q.in_flight() == 90 && !q.empty(0));
+  q.finished(70);
+  q.pop_admitted(50, 0);
+  q.pop_admitted(5, 1);
+  CHECK(q.empty() && q.empty(0) && q.empty(1) && q.peak() == 90 && q.in_flight() == 75);
+}
+
+// A host-staged request whose rounded size exceeds the arena can never get a
+// block, even when its unrounded size is within the budget.
+static void arena_can_hold() {
+  gfft::Arena a(1000);  // 768 usable
+  CHECK(a.can_hold(768) && a.can_hold(700) && !a.can_hold(769) && !a.can_hold(1000));
+  gfft::Arena b(0);
+  CHECK(!b.can_hold(1));
+}
+
+static void stream_choice() {
+  const size_t depth[4] = {1, 0, 2, 0}, bytes[4] = {50, 90, 10, 5};
+  // no reservation, no depth limit: least bytes
+  CHECK(gfft::pick_stream(depth, bytes, 4, false, false, 0) == 3);
+  // the last stream is reserved: a large request takes the least loaded of 0..2
+  CHECK(gfft::pick_stream(depth, bytes, 4, true, false, 0) == 2);
+  CHECK(gfft::pick_stream(depth, bytes, 4, true, true, 0) == 3);
+  // depth 2: stream 2 is full
+  CHECK(gfft::pick_stream(depth, bytes, 4, true, false, 2) == 0);
+  // depth 1: only streams 1 and 3 have room
+  CHECK(gfft::pick_stream(depth, bytes, 4, true, false, 1) == 1);
+  const size_t full[2] = {2, 2}, b2[2] = {1, 1};
+  CHECK(gfft::pick_stream(full, b2, 2, false, false, 2) == -1);  // stays in its FIFO
+  const size_t tie[3] = {0, 0, 0}, bt[3] = {7, 7, 7};
+  CHECK(gfft::pick_stream(tie, bt, 3, false, false, 2) == 0);
+}
+
 int main() {
+  admission_two_queues();
+  arena_can_hold();
+  stream_choice();
   arena_basics();
   arena_coalesces_both_sides();
   arena_random();

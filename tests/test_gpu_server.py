@@ -200,6 +200,7 @@ def test_shared_queue_streams_give_the_same_results():
     out = {}
     for own in (True, False):
         work = [x.clone() for x in inputs]
+        torch.cuda.synchronize()  # the copies are complete before any server stream reads them
         with S.Server(budget_bytes=2 << 30, nstreams=4, own_queues=own) as srv:
             for (d, p, dr), w in zip(reqs, work):
                 srv.submit(d, p, dr, w)
@@ -208,3 +209,74 @@ def test_shared_queue_streams_give_the_same_results():
         out[own] = work
     for a, b in zip(out[True], out[False]):
         assert torch.equal(a, b)
+
+
+def test_producer_stream_orders_the_request_after_its_data():
+    """A device request names the stream that writes its array: the server's
+    stream waits for that work (queued before submit, not yet run) without
+    any host synchronisation, on non-blocking server streams."""
+    import torch
+    dims, prec = (128, 128, 128), "f32"
+    x = make_input(torch, *dims, prec, S.R2C, seed=150)
+    ref = alone(torch, dims, prec, S.R2C, x)
+    side = torch.cuda.Stream()
+    work = torch.zeros_like(x)
+    a = torch.randn(4096, 4096, device="cuda:0")
+    torch.cuda.synchronize()
+    say("case producer stream, 128^3 f32 written on a side stream")
+    with S.Server(budget_bytes=1 << 30, nstreams=2, own_queues=False) as srv:
+        with torch.cuda.stream(side):
+            for _ in range(8):          # several ms of work ahead of the copy
+                a = a @ a
+                a = a / a.norm()
+            work.copy_(x)
+        srv.submit(dims, prec, S.R2C, work, producer=side)
+        res = srv.drain(timeout_s=60)
+    assert len(res) == 1 and res[0]["status"] == 0
+    assert torch.equal(work, ref)
+
+
+def test_stream_depth_caps_requests_per_stream():
+    """With depth 1 each stream holds one unfinished request at a time, so
+    the requests of one stream never overlap in [admitted, done] and later
+    requests wait in the FIFO (admitted - arrival grows); results are those
+    of the plan API.  The started stamp lies between admission and done."""
+    import torch
+    dims, prec = (192, 192, 192), "f32"
+    base = make_input(torch, *dims, prec, S.R2C, seed=160)
+    ref = alone(torch, dims, prec, S.R2C, base)
+    bufs = [base.clone() for _ in range(10)]
+    torch.cuda.synchronize()
+    say("case stream depth 1, 10 requests of 192^3 f32 on 2 streams")
+    with S.Server(budget_bytes=8 << 30, nstreams=2) as srv:
+        srv.set_small_lane(0)
+        srv.set_stream_depth(1)
+        for i, b in enumerate(bufs):
+            srv.submit(dims, prec, S.R2C, b, tag=i)
+        res = srv.drain(timeout_s=120)
+    assert len(res) == 10 and all(r["status"] == 0 for r in res)
+    for st in (0, 1):
+        rs = sorted((r for r in res if r["stream"] == st), key=lambda r: r["admitted"])
+        for a, b in zip(rs, rs[1:]):
+            assert b["admitted"] >= a["done"], (a, b)
+    assert max(r["admitted"] - r["arrival"] for r in res) > 0
+    for r in res:
+        assert r["admitted"] <= r["started"] <= r["done"], r
+    for b in bufs:
+        assert torch.equal(b, ref)
+
+
+def test_staged_request_that_no_arena_block_can_hold_is_rejected():
+    """Host-staged 5^3 FP32 is 600 B (768 B in 256-B blocks); a 700-B budget
+    gives a 512-B arena: rejected at submission instead of blocking the
+    FIFO for good."""
+    import torch
+    host = torch.zeros(5, 5, 6).pin_memory()
+    out = torch.zeros_like(host).pin_memory()
+    assert S.request_bytes(5, 5, 5, "f32") == 600
+    with S.Server(budget_bytes=700, nstreams=1, staging=True) as srv:
+        with pytest.raises(G.GfftError) as e:
+            srv.submit((5, 5, 5), "f32", S.R2C, host, host_out=out, device=False)
+        assert e.value.status == 10
+        srv.wait(10)
+        assert srv.stats()["rejected"] == 1

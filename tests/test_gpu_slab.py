@@ -132,3 +132,74 @@ def test_p8_share_of_1024_cube_real_shape():
     assert err_max <= 1e-6
     del data, send
     torch.cuda.empty_cache()
+
+
+def _rand_chunks(torch, n_complex, dt, seed):
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(seed)
+    return torch.rand(2 * n_complex, dtype=dt, device="cuda:0", generator=g) * 2 - 1
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_p8_share_finish_start_local_real_shape(prec):
+    """The other three local halves of rank 3's share of 1024^3 at p = 8, at
+    their real shapes, on synthetic exchange blocks, against FP64 numpy
+    block by block:
+      r2c_finish: received chunks (p, Ny/p, Nx/p, K) = (8, 128, 128, 513),
+        chunk s holding x-planes s*128.. -> x C2C forward -> (1024, 128, 513);
+      c2r_start:  a (1024, 128, 513) y-slab -> x C2C inverse -> packed into
+        chunk s = x-planes s*128.. in (Ny/p, Nx/p, K) order;
+      c2r_local:  received chunks, chunk s holding y rows s*128.. -> y C2C
+        inverse + z C2R (numpy irfft convention) -> (128, 1024, 1026) real."""
+    import torch
+    nx = ny = nz = 1024
+    procs, rank = 8, 3
+    nxl, nyl, K = nx // procs, ny // procs, O.kdim(nz)
+    dt = torch.float32 if prec == "f32" else torch.float64
+    ncx = nx * nyl * K                      # complex elements of one exchange buffer
+    say(f"case p=8 share finish/start/local {prec}")
+    with G.Plan(nx, ny, nz, procs=procs, rank=rank, precision=prec) as p:
+        # r2c_finish
+        recv = _rand_chunks(torch, ncx, dt, 11)
+        out = torch.empty_like(recv)
+        p.r2c_finish(recv, out)
+        torch.cuda.synchronize()
+        r = recv.view(-1, 2).cpu().numpy().astype(np.float64)
+        rc = (r[:, 0] + 1j * r[:, 1]).reshape(procs, nyl, nxl, K)
+        got = G.complex_view(out, nx, nyl, nz).cpu().numpy()
+        a = np.transpose(rc, (0, 2, 1, 3)).reshape(nx, nyl, K)   # x = s*128 + x_l
+        err = max(O.rel_l2(got[:, :, k0:k0 + 64], np.fft.fft(a[:, :, k0:k0 + 64], axis=0))
+                  for k0 in range(0, K, 64))
+        say(f"  r2c_finish max block rel L2 {err:.2e}")
+        assert err <= TOL[prec]
+        # c2r_start: reuse `out` (a y-slab) as input
+        yslab = G.complex_view(out, nx, nyl, nz).cpu().numpy().astype(np.complex128)
+        send = torch.empty_like(recv)
+        p.c2r_start(out, send)
+        torch.cuda.synchronize()
+        s_ = send.view(-1, 2).cpu().numpy().astype(np.float64)
+        sc = (s_[:, 0] + 1j * s_[:, 1]).reshape(procs, nyl, nxl, K)
+        want = np.fft.ifft(yslab, axis=0) * nx                       # unnormalised inverse
+        wc = np.transpose(want.reshape(procs, nxl, nyl, K), (0, 2, 1, 3))
+        err = max(O.rel_l2(sc[s], wc[s]) for s in range(procs))
+        say(f"  c2r_start max chunk rel L2 {err:.2e}")
+        assert err <= TOL[prec]
+        del yslab, want, wc, sc, s_, a, rc, r, got
+        # c2r_local
+        recv2 = _rand_chunks(torch, ncx, dt, 12)
+        data = torch.empty((nxl, ny, 2 * K), dtype=dt, device="cuda:0")
+        r = recv2.view(-1, 2).cpu().numpy().astype(np.float64)
+        rc = (r[:, 0] + 1j * r[:, 1]).reshape(procs, nyl, nxl, K)
+        b = np.transpose(rc, (2, 0, 1, 3)).reshape(nxl, ny, K)       # y = s*128 + y_l
+        p.c2r_local(recv2, data)
+        torch.cuda.synchronize()
+        got = data.cpu().numpy()[:, :, :nz]
+        err = 0.0
+        for x0 in range(0, nxl, 16):
+            ref = np.fft.irfft2(b[x0:x0 + 16], s=(ny, nz), axes=(1, 2)) * (ny * nz)
+            err = max(err, O.rel_l2(got[x0:x0 + 16], ref))
+        say(f"  c2r_local max block rel L2 {err:.2e}")
+        assert err <= TOL[prec]
+    del recv, out, send, recv2, data
+    torch.cuda.empty_cache()
