@@ -260,9 +260,19 @@ def batch_bench(args):
     ms_batch_eager = timed(lambda: batch.exec(G.ROUND_TRIP, stream))
     # every execution above was a normalised round trip: the fields came back
     torch.cuda.synchronize()
+    status = batch.status()
     rt_err = max(float((b - r)[..., :d[2]].abs().max() / r[..., :d[2]].abs().max())
                  for b, r, d in zip(bufs, ref, dims))
     tiles, launches = batch.tiles(G.ROUND_TRIP)
+    # each scheduler with its tuned choices, as one graph launch
+    sched_ms = {}
+    for name, sch in (("persistent", G.Batch.PERSISTENT), ("levels", G.Batch.LEVELS)):
+        print(f"batch: {name} scheduler, graph path", file=sys.stderr, flush=True)
+        batch.set_scheduler(sch)
+        sched_ms[name] = {"graph_ms": round(timed(lambda: batch.exec_graph(G.ROUND_TRIP, stream)), 5),
+                          "launches": batch.tiles(G.ROUND_TRIP)[1]}
+    batch.set_scheduler(G.Batch.PERSISTENT if chosen["scheduler"] == "persistent" else G.Batch.LEVELS)
+    status = max(status, batch.status())
     counts = {n: sum(1 for d in dims if d[0] == n) for n in BATCH_SIZES}
     out = {"metric": "batched 256 mixed-size FP32 R2C+C2R throughput vs sequential plain-method launches",
            "value": round(args.batch / (ms_batch * 1e-3), 1), "unit": "transforms/s (round trips)",
@@ -275,7 +285,13 @@ def batch_bench(args):
            "speedup_vs_plain": round(ms_plain / ms_batch, 3),
            "gflops_batch": round(flops / (ms_batch * 1e-3) / 1e9, 1),
            "alg_GBps_12S": round(12 * s_bytes / (ms_batch * 1e-3) / 1e9, 1),
-           "roundtrip_max_rel_err": rt_err, "variants": {str(k): v for k, v in chosen.items()}}
+           "roundtrip_max_rel_err": rt_err, "persistent_status": status,
+           "plan": batch.plan_info(G.ROUND_TRIP),
+           "tuned": {"scheduler": chosen["scheduler"], "tuned_eager_ms": chosen["tuned_ms"],
+                     "persistent": {"set": chosen["persist"]["set"], "team_factor": chosen["persist"]["team_factor"],
+                                    "variants": {str(k): v for k, v in chosen["persist"]["variants"].items()}},
+                     "level_variants": {str(k): v for k, v in chosen["variants"].items()}},
+           "schedulers": sched_ms}
     batch.close()
     for p in plans.values():
         p.close()
@@ -725,7 +741,9 @@ def main():
         log("batched mode: 256 mixed cubes")
         b = batch_bench(argparse.Namespace(batch=256, steps=20, warmup=3))
         out["batch"] = {k: b[k] for k in ("plain_ms", "batch_graph_ms", "batch_eager_ms", "speedup_vs_plain",
-                                          "gflops_batch", "roundtrip_max_rel_err", "variants")}
+                                          "gflops_batch", "roundtrip_max_rel_err", "persistent_status", "tuned",
+                                          "schedulers", "plan")}
+        out["batch"]["launches"] = b["config"]["launches"]
         out["batch"]["workload"] = b["config"]["workload"]
         log("serving: device-resident requests")
         out["serving"] = serve_bench(5000, staged=False, load=0.8)

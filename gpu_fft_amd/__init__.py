@@ -20,11 +20,13 @@ import os
 __all__ = [
     "GfftError", "Plan", "lib", "lib_path", "query_plan", "kdim", "alloc_inplace",
     "real_view", "complex_view", "F32", "F64", "tune_cache_text", "load_tune_cache_text", "clear_tune_cache",
-    "save_tune_cache", "load_tune_cache",
+    "save_tune_cache", "load_tune_cache", "ERR_TIMEOUT", "ERR_ABORTED",
 ]
 
 F32 = 0
 F64 = 1
+ERR_TIMEOUT = 11  # GFFT_ERR_TIMEOUT
+ERR_ABORTED = 12  # GFFT_ERR_ABORTED
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -341,6 +343,17 @@ def _bind_batch(L):
     L.gfft_batch_set_variant.argtypes = [vp, i64, ctypes.c_int]
     L.gfft_batch_variant.restype = ctypes.c_int
     L.gfft_batch_variant.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_int)]
+    ip = ctypes.POINTER(ctypes.c_int)
+    for name, args in (("gfft_batch_set_scheduler", [vp, ctypes.c_int]), ("gfft_batch_scheduler", [vp, ip]),
+                       ("gfft_batch_set_persist", [vp, ctypes.c_int, ctypes.c_int]),
+                       ("gfft_batch_persist", [vp, ip, ip, ip]),
+                       ("gfft_batch_set_wait_limit", [vp, ctypes.c_double]),
+                       ("gfft_batch_set_abort", [vp, ctypes.c_int]), ("gfft_batch_status", [vp, ip]),
+                       ("gfft_batch_plan_info", [vp, ctypes.c_int] + [ctypes.POINTER(ctypes.c_uint32)] * 3),
+                       ("gfft_batch_tuned_ms", [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)])):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = args
     L._gfft_batch_bound = True
     return L
 
@@ -390,12 +403,69 @@ class Batch:
         _check(lib().gfft_batch_exec_graph(self.handle, int(mode), _stream_handle(stream)), "gfft_batch_exec_graph")
 
     def tune(self, mode=ROUND_TRIP, reps: int = 3, stream=None) -> dict:
-        """Time every variant of every size class on the current data
-        (destroyed) and keep the fastest; returns {edge: variant}."""
+        """Autotune both schedulers on the current data (destroyed) and keep
+        the faster; returns describe() with both tuned times."""
         _check(lib().gfft_batch_tune(self.handle, int(mode), int(reps), _stream_handle(stream)), "gfft_batch_tune")
-        return self.variants()
+        p, l = ctypes.c_float(), ctypes.c_float()
+        _check(lib().gfft_batch_tuned_ms(self.handle, ctypes.byref(p), ctypes.byref(l)), "gfft_batch_tuned_ms")
+        d = self.describe()
+        d["tuned_ms"] = {"persistent": round(p.value, 5), "levels": round(l.value, 5)}
+        d["persist"] = self.persist()
+        d["variants"] = self.variants()
+        return d
 
     PLAIN = 3  # variant: the class's transforms through single-transform plans
+    PERSISTENT, LEVELS = 0, 1  # schedulers
+
+    def set_scheduler(self, scheduler: int) -> None:
+        """PERSISTENT (default): one persistent launch per execution;
+        LEVELS: one launch per (size, axis pass)."""
+        _check(lib().gfft_batch_set_scheduler(self.handle, int(scheduler)), "gfft_batch_set_scheduler")
+
+    def scheduler(self) -> int:
+        v = ctypes.c_int()
+        _check(lib().gfft_batch_scheduler(self.handle, ctypes.byref(v)), "gfft_batch_scheduler")
+        return v.value
+
+    def set_persist(self, set_index: int, team_factor: int) -> None:
+        """Persistent scheduler: variant set (0..1) and team factor (a
+        transform gets a team of workgroups above that many slot shares)."""
+        _check(lib().gfft_batch_set_persist(self.handle, int(set_index), int(team_factor)), "gfft_batch_set_persist")
+
+    def persist(self) -> dict:
+        s, c = ctypes.c_int(), ctypes.c_int()
+        v = (ctypes.c_int * 8)()
+        _check(lib().gfft_batch_persist(self.handle, ctypes.byref(s), ctypes.byref(c), v), "gfft_batch_persist")
+        sizes = [32, 48, 64, 96, 128, 160, 192, 256]
+        present = {d[2] for d in self.dims}
+        return {"set": s.value, "team_factor": c.value,
+                "variants": {n: v[i] for i, n in enumerate(sizes) if n in present}}
+
+    def plan_info(self, mode=ROUND_TRIP) -> dict:
+        """Persistent plan of ``mode``: slots (workgroups), teams, members."""
+        a, b, c = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().gfft_batch_plan_info(self.handle, int(mode), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+               "gfft_batch_plan_info")
+        return {"slots": a.value, "teams": b.value, "members": c.value}
+
+    def describe(self) -> dict:
+        if self.scheduler() == self.PERSISTENT:
+            return {"scheduler": "persistent", **self.persist()}
+        return {"scheduler": "levels", "variants": self.variants()}
+
+    def set_wait_limit(self, seconds: float) -> None:
+        _check(lib().gfft_batch_set_wait_limit(self.handle, float(seconds)), "gfft_batch_set_wait_limit")
+
+    def set_abort(self, abort: bool) -> None:
+        _check(lib().gfft_batch_set_abort(self.handle, 1 if abort else 0), "gfft_batch_set_abort")
+
+    def status(self) -> int:
+        """Synchronise, then the sticky outcome of the persistent launches
+        since the last call (0 = ok, 11 = a wait timed out, 12 = aborted),
+        which is cleared."""
+        v = ctypes.c_int()
+        _check(lib().gfft_batch_status(self.handle, ctypes.byref(v)), "gfft_batch_status")
+        return v.value
 
     def set_variant(self, edge: int, variant: int) -> None:
         """Variant of a size class: 0-2 = level kernels (radix order, points

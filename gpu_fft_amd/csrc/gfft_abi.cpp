@@ -52,7 +52,8 @@ const char* gfft_strerror(int status) {
     case GFFT_ERR_DEVICE_MISMATCH: return "current HIP device differs from the plan's device";
     case GFFT_ERR_BUFFER_TOO_SMALL: return "text buffer too small";
     case GFFT_ERR_OVER_BUDGET: return "request larger than the server's whole HBM budget";
-    case GFFT_ERR_TIMEOUT: return "timed out waiting for the server to drain";
+    case GFFT_ERR_TIMEOUT: return "timed out (server drain, or a bounded wait of a persistent batch launch)";
+    case GFFT_ERR_ABORTED: return "a persistent batch launch gave up its waits on the host's abort request";
     default: return "unknown status";
   }
 }

@@ -27,149 +27,14 @@
 #include <tuple>
 #include <vector>
 
-#include "gfft_kernels.h"
+#include "gfft_batch_tiles.h"
 #include "gfft_plan.h"
 #include "gpufft/gpufft.h"
 
 namespace gfft {
 namespace {
 
-constexpr int kBT = 256;  // threads per workgroup, every tile kind
-constexpr int kClasses = 8;
-constexpr int kVariants = 3;
-constexpr int kPlainVariant = kVariants;  // per-transform plans
-constexpr int kMaxLevels = 6;
-constexpr int kLanes = 4;  // internal streams of the eager path
-
-// Variant V of size class C: radix order and points per thread P; with
-// TPT = N/P threads per transform a tile carries W = 256/TPT row pairs (z)
-// or columns (y, x).
-template <int C, int V>
-struct BVar;
-#define GFFT_BVAR(C, V, P_, ...)     \
-  template <>                        \
-  struct BVar<C, V> {                \
-    using SC = Sched<__VA_ARGS__>;   \
-    static constexpr int P = P_;     \
-  };
-GFFT_BVAR(0, 0, 8, 8, 4)
-GFFT_BVAR(0, 1, 8, 4, 8)
-GFFT_BVAR(0, 2, 4, 8, 4)
-GFFT_BVAR(1, 0, 12, 4, 4, 3)
-GFFT_BVAR(1, 1, 12, 3, 4, 4)
-GFFT_BVAR(1, 2, 6, 4, 4, 3)
-GFFT_BVAR(2, 0, 8, 8, 8)
-GFFT_BVAR(2, 1, 4, 4, 4, 4)
-GFFT_BVAR(2, 2, 16, 8, 8)
-GFFT_BVAR(3, 0, 24, 8, 4, 3)
-GFFT_BVAR(3, 1, 12, 8, 4, 3)
-GFFT_BVAR(3, 2, 24, 3, 8, 4)
-GFFT_BVAR(4, 0, 8, 8, 4, 4)
-GFFT_BVAR(4, 1, 8, 4, 4, 8)
-GFFT_BVAR(4, 2, 16, 8, 4, 4)
-GFFT_BVAR(5, 0, 20, 4, 4, 2, 5)
-GFFT_BVAR(5, 1, 10, 8, 4, 5)
-GFFT_BVAR(5, 2, 20, 8, 4, 5)
-GFFT_BVAR(6, 0, 24, 8, 8, 3)
-GFFT_BVAR(6, 1, 12, 8, 8, 3)
-GFFT_BVAR(6, 2, 24, 3, 8, 8)
-GFFT_BVAR(7, 0, 8, 8, 8, 4)
-GFFT_BVAR(7, 1, 16, 8, 8, 4)
-GFFT_BVAR(7, 2, 8, 4, 8, 8)
-#undef GFFT_BVAR
-
-constexpr int kClassN[kClasses] = {32, 48, 64, 96, 128, 160, 192, 256};
-
-template <int C, int V>
-struct BCfg {
-  using SC = typename BVar<C, V>::SC;
-  static constexpr int N = SC::N;
-  static_assert(N == kClassN[C], "variant size");
-  static constexpr int P = BVar<C, V>::P;
-  static constexpr int TPT = N / P;
-  static constexpr int W = kBT / TPT;  // pairs per rows tile = columns per cols tile
-  static_assert(W * TPT == kBT, "tile must use all 256 threads");
-  using R2C = RowsR2C<float, SC, P, W>;
-  using C2R = RowsC2R<float, SC, P, W>;
-  using FWD = ColsPass<float, false, SC, P, W>;
-  using INV = ColsPass<float, true, SC, P, W>;
-  static constexpr int lds_entries() {
-    const int rows = W * R2C::LROW;
-    const int cols = N * W;
-    return rows > cols ? rows : cols;
-  }
-};
-
-int class_of(int64_t n) {
-  for (int c = 0; c < kClasses; ++c)
-    if (kClassN[c] == n) return c;
-  return -1;
-}
-
-// One transform of the batch.
-struct BXform {
-  float* data;       // in-place array (Nx, Ny, 2K)
-  int32_t cls[3];    // size class of the z, y, x pass (-1: length-1 strided axis)
-  int32_t K;
-  int32_t ny;
-  int64_t nrows;     // Nx * Ny
-  uint32_t ncols_y;  // Nx * K
-  uint32_t ncols_x;  // Ny * K
-  float scale_r2c, scale_c2r;
-};
-
-// One level launch: one axis pass over every transform of one class.
-struct LevelArgs {
-  const BXform* xf;        // the class's transforms
-  const uint32_t* prefix;  // [count + 1] tile prefix sums of this level
-  const cx<float>* tw;     // twiddle table of the class's variant
-  int32_t count;
-  int32_t axis;            // 0 = z rows, 1 = y columns, 2 = x columns
-  int32_t inv;
-  uint32_t total;          // tiles of this level = prefix[count]
-};
-
-__device__ __forceinline__ ColMap dmap(uint32_t inner, int64_t outer, int64_t rlo) {
-  ColMap m;
-  m.inner = inner;
-  m.nsub = 0x7fffffff;
-  m.outer = outer;
-  m.rlo = rlo;
-  m.rhi = 0;
-  return m;
-}
-
-template <typename F>
-__device__ __forceinline__ void run_tile(const BXform& x, int axis, bool inv, uint32_t tile,
-                                         const cx<float>* tw, cx<float>* smem) {
-  if (axis == 0) {
-    RowGeom g;
-    g.nrows = x.nrows;
-    g.K = x.K;
-    g.scale = inv ? x.scale_c2r : 1.0;
-    if (inv)
-      F::C2R::tile(x.data, g, tw, (int64_t)tile * F::W, smem);
-    else
-      F::R2C::tile(x.data, g, tw, (int64_t)tile * F::W, smem);
-    return;
-  }
-  ColGeom g;
-  if (axis == 1) {
-    g.ncols = x.ncols_y;
-    g.in = g.out = dmap((uint32_t)x.K, (int64_t)x.ny * x.K, x.K);
-  } else {
-    g.ncols = x.ncols_x;
-    g.in = g.out = dmap(x.ncols_x, 0, (int64_t)x.ncols_x);
-  }
-  // the forward scale belongs to the last R2C pass (x, or y/z when x is length 1)
-  const bool last_fwd = !inv && (axis == 2 || (axis == 1 && x.cls[2] < 0));
-  g.scale = last_fwd ? x.scale_r2c : 1.0;
-  cx<float>* d = reinterpret_cast<cx<float>*>(x.data);
-  if (inv)
-    F::INV::tile(d, d, g, tw, tile, smem);
-  else
-    F::FWD::tile(d, d, g, tw, tile, smem);
-}
+using namespace batch;
 
 // Workgroups stride over the level's tiles; tile k belongs to the transform
 // t with prefix[t] <= k < prefix[t + 1].
@@ -241,7 +106,7 @@ const VarInfo& var(int c, int v) {
 }  // namespace
 }  // namespace gfft
 
-using gfft::BXform;
+using gfft::batch::BXform;
 
 // Transforms are grouped by size class; group c occupies positions
 // [off[c], off[c] + n[c]) of the device descriptor array.
@@ -252,31 +117,49 @@ struct gfft_batch_s {
   std::vector<size_t> bytes;   // data bytes per transform, caller order
   std::vector<int> pos;        // caller index -> sorted position
   std::vector<BXform> hx;      // sorted by class
-  int off[gfft::kClasses] = {}, n[gfft::kClasses] = {};
-  int var[gfft::kClasses] = {};  // chosen variant per class
+  int off[gfft::batch::kClasses] = {}, n[gfft::batch::kClasses] = {};
+  int var[gfft::batch::kClasses] = {};  // chosen variant per class
   BXform* d_xf = nullptr;
   uint32_t* d_prefix = nullptr;  // [class][variant][mode][level][n_c + 1], packed
-  size_t prefix_off[gfft::kClasses][gfft::kVariants][3] = {};
-  uint32_t level_tiles[gfft::kClasses][gfft::kVariants][3][gfft::kMaxLevels] = {};
-  void* d_tw[gfft::kClasses][gfft::kVariants] = {};
-  int occ[gfft::kClasses][gfft::kVariants] = {};  // resident workgroups per CU
-  int lane[gfft::kClasses] = {};  // internal stream of the class's chain (eager path)
+  size_t prefix_off[gfft::batch::kClasses][gfft::batch::kVariants][3] = {};
+  uint32_t level_tiles[gfft::batch::kClasses][gfft::batch::kVariants][3][gfft::batch::kMaxLevels] = {};
+  void* d_tw[gfft::batch::kClasses][gfft::batch::kVariants] = {};
+  int occ[gfft::batch::kClasses][gfft::batch::kVariants] = {};  // resident workgroups per CU
+  float tuned_ms[2] = {-1.0f, -1.0f};  // last tune: ms per execution, persistent and levels
+  int lane[gfft::batch::kClasses] = {};  // internal stream of the class's chain (eager path)
   int cus = 0;
   bool data_set = false;
-  hipStream_t ws[gfft::kLanes] = {};
+  hipStream_t ws[gfft::batch::kLanes] = {};
   hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_join[gfft::kLanes] = {};
+  hipEvent_t ev_join[gfft::batch::kLanes] = {};
   hipGraph_t graph[3] = {};
   hipGraphExec_t gexec[3] = {};
   // single-transform plans (one per distinct geometry) of the plain variant
   std::map<std::tuple<int64_t, int64_t, int64_t>, std::unique_ptr<gfft::PlanCore>> plans;
   std::vector<const gfft::PlanCore*> xplan;  // per sorted position
+  // persistent execution (default): one launch per mode following a plan
+  int sched = GFFT_BATCH_PERSISTENT;
+  int pset = 0, pteam = 2;            // variant set, team factor (a team above pteam slot shares of work)
+  struct DevPlan {
+    gfft::PSeg* seg = nullptr;
+    gfft::PTeam* team = nullptr;
+    gfft::PSlot* slot = nullptr;
+    gfft::PEntry* entry = nullptr;
+    uint32_t nslots = 0, claim_words = 0, tiles = 0, teams_k2 = 0, members = 0;
+  } plan[3];
+  uint32_t* d_ctr = nullptr;          // counter words, sized for the largest plan
+  size_t ctr_words = 0;
+  uint32_t* h_abort = nullptr;        // pinned host word read by the kernel's waits
+  double wait_limit_s = 2.0;
+  int wall_khz = 0;                   // device wall-clock rate
+  size_t plds = 0;                    // dynamic LDS of the current set for the classes present
+  int pocc = 1;                       // resident persistent workgroups per CU
 };
 
 namespace {
 
-const int kModeAxis[3][gfft::kMaxLevels] = {{0, 1, 2}, {2, 1, 0}, {0, 1, 2, 2, 1, 0}};
-const int kModeInv[3][gfft::kMaxLevels] = {{0, 0, 0}, {1, 1, 1}, {0, 0, 0, 1, 1, 1}};
+const int kModeAxis[3][gfft::batch::kMaxLevels] = {{0, 1, 2}, {2, 1, 0}, {0, 1, 2, 2, 1, 0}};
+const int kModeInv[3][gfft::batch::kMaxLevels] = {{0, 0, 0}, {1, 1, 1}, {0, 0, 0, 1, 1, 1}};
 const int kModeLevels[3] = {3, 3, 6};
 
 uint32_t tiles_of(const BXform& x, int axis, int w) {
@@ -303,30 +186,44 @@ void drop_graphs(gfft_batch_s* b) {
 }
 
 bool any_plain(const gfft_batch_s* b) {
-  for (int c = 0; c < gfft::kClasses; ++c)
-    if (b->n[c] && b->var[c] == gfft::kPlainVariant) return true;
+  for (int c = 0; c < gfft::batch::kClasses; ++c)
+    if (b->n[c] && b->var[c] == gfft::batch::kPlainVariant) return true;
   return false;
+}
+
+void free_plans(gfft_batch_s* b) {
+  for (auto& p : b->plan) {
+    if (p.seg) (void)hipFree(p.seg);
+    if (p.team) (void)hipFree(p.team);
+    if (p.slot) (void)hipFree(p.slot);
+    if (p.entry) (void)hipFree(p.entry);
+    p = gfft_batch_s::DevPlan{};
+  }
 }
 
 void release(gfft_batch_s* b) {
   drop_graphs(b);
+  if (b->d_ctr || b->plan[0].seg || b->plan[1].seg || b->plan[2].seg) (void)hipDeviceSynchronize();
+  free_plans(b);
+  if (b->d_ctr) (void)hipFree(b->d_ctr);
+  if (b->h_abort) (void)hipHostFree(b->h_abort);
   for (auto& kv : b->plans) gfft::free_twiddles(*kv.second);
   b->plans.clear();
-  for (int i = 0; i < gfft::kLanes; ++i) {
+  for (int i = 0; i < gfft::batch::kLanes; ++i) {
     if (b->ev_join[i]) (void)hipEventDestroy(b->ev_join[i]);
     if (b->ws[i]) (void)hipStreamDestroy(b->ws[i]);
   }
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->d_xf) (void)hipFree(b->d_xf);
   if (b->d_prefix) (void)hipFree(b->d_prefix);
-  for (int c = 0; c < gfft::kClasses; ++c)
-    for (int v = 0; v < gfft::kVariants; ++v)
+  for (int c = 0; c < gfft::batch::kClasses; ++c)
+    for (int v = 0; v < gfft::batch::kVariants; ++v)
       if (b->d_tw[c][v]) (void)hipFree(b->d_tw[c][v]);
 }
 
-gfft::LevelArgs level_args(const gfft_batch_s* b, int c, int mode, int l) {
+gfft::batch::LevelArgs level_args(const gfft_batch_s* b, int c, int mode, int l) {
   const int v = b->var[c];
-  gfft::LevelArgs a{};
+  gfft::batch::LevelArgs a{};
   a.xf = b->d_xf + b->off[c];
   a.prefix = b->d_prefix + b->prefix_off[c][v][mode] + (size_t)l * (b->n[c] + 1);
   a.tw = static_cast<const gfft::cx<float>*>(b->d_tw[c][v]);
@@ -359,9 +256,9 @@ int launch_plain(const gfft_batch_s* b, int c, int mode, hipStream_t s) {
 
 // The level chain of one class on one stream.
 int launch_chain(const gfft_batch_s* b, int c, int mode, hipStream_t s) {
-  if (b->var[c] == gfft::kPlainVariant) return launch_plain(b, c, mode, s);
+  if (b->var[c] == gfft::batch::kPlainVariant) return launch_plain(b, c, mode, s);
   for (int l = 0; l < kModeLevels[mode]; ++l) {
-    const gfft::LevelArgs a = level_args(b, c, mode, l);
+    const gfft::batch::LevelArgs a = level_args(b, c, mode, l);
     if (a.total == 0) continue;
     if (gfft::var(c, b->var[c]).launch(dim3(level_grid(b, c, a.total)), a, s) != hipSuccess) return GFFT_ERR_HIP;
   }
@@ -387,6 +284,147 @@ int launch_eager(gfft_batch_s* b, int mode, hipStream_t s) {
     if (hipEventRecord(b->ev_join[i], b->ws[i]) != hipSuccess) return GFFT_ERR_HIP;
     if (hipStreamWaitEvent(s, b->ev_join[i], 0) != hipSuccess) return GFFT_ERR_HIP;
   }
+  return GFFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent execution: the plans of the three modes for the current set and
+// team factor, for as many slots as the device holds workgroups of the
+// set's kernel at once; uploaded once, read by every launch.
+// ---------------------------------------------------------------------------
+template <typename T>
+int upload_vec(T** d, const std::vector<T>& v) {
+  if (hipMalloc(d, sizeof(T) * std::max<size_t>(v.size(), 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    *d = nullptr;
+    return GFFT_ERR_OUT_OF_MEMORY;
+  }
+  if (!v.empty() && hipMemcpy(*d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return GFFT_ERR_HIP;
+  return GFFT_OK;
+}
+
+int build_plans(gfft_batch_s* b) {
+  using namespace gfft;
+  using namespace gfft::batch;
+  if (hipDeviceSynchronize() != hipSuccess) return GFFT_ERR_HIP;  // earlier launches read the old plans
+  free_plans(b);
+  int w[kClasses];
+  bool present[kClasses] = {};
+  for (int c = 0; c < kClasses; ++c) {
+    w[c] = var(c, kSetVariant[b->pset][c]).W;
+    present[c] = b->n[c] > 0;
+  }
+  b->plds = persist_lds_bytes(b->pset, present);
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, persist_kernel(b->pset), kBT, b->plds) != hipSuccess ||
+      occ < 1)
+    occ = 1;
+  b->pocc = occ;
+  const uint32_t slots = (uint32_t)std::max(1, b->cus * occ);
+  size_t need = 0;
+  for (int m = 0; m < 3; ++m) {
+    PLevelSpec lv[kMaxLevels];
+    for (int l = 0; l < kModeLevels[m]; ++l) lv[l] = PLevelSpec{kModeAxis[m][l], kModeInv[m][l]};
+    auto tiles = [&](uint32_t t, int l) {
+      const BXform& x = b->hx[t];
+      const int c = x.cls[kModeAxis[m][l]];
+      return c < 0 ? 0u : tiles_of(x, kModeAxis[m][l], w[c]);
+    };
+    const PPlan pl = build_plan(
+        (uint32_t)b->count, kModeLevels[m], lv, tiles, [&](uint32_t t, int l) { return b->hx[t].cls[kModeAxis[m][l]]; },
+        [&](uint32_t t) {
+          const BXform& x = b->hx[t];
+          double levels = 0;
+          for (int l = 0; l < kModeLevels[m]; ++l) levels += tiles(t, l) ? 1.0 : 0.0;
+          return (double)x.nrows * x.K * levels;
+        },
+        slots, (double)b->pteam);
+    if (check_plan(pl) != 0) return GFFT_ERR_INVALID_ARG;  // cannot happen: build_plan's construction
+    gfft_batch_s::DevPlan& dp = b->plan[m];
+    int st = upload_vec(&dp.seg, pl.segs);
+    if (st == GFFT_OK) st = upload_vec(&dp.team, pl.teams);
+    if (st == GFFT_OK) st = upload_vec(&dp.slot, pl.slots);
+    if (st == GFFT_OK) st = upload_vec(&dp.entry, pl.entries);
+    if (st != GFFT_OK) {
+      free_plans(b);
+      return st;
+    }
+    dp.nslots = (uint32_t)pl.slots.size();
+    dp.claim_words = pl.claim_words;
+    for (const PSeg& sg : pl.segs) dp.tiles += sg.ntiles;
+    for (const PTeam& tm : pl.teams)
+      if (tm.k >= 2) {
+        ++dp.teams_k2;
+        dp.members += tm.k;
+      }
+    need = std::max(need, counter_words(pl));
+  }
+  if (need > b->ctr_words) {  // counters for the largest plan, zeroed
+    if (b->d_ctr) (void)hipFree(b->d_ctr);
+    b->d_ctr = nullptr;
+    b->ctr_words = 0;
+    if (hipMalloc(&b->d_ctr, sizeof(uint32_t) * need) != hipSuccess) {
+      (void)hipGetLastError();
+      free_plans(b);
+      return GFFT_ERR_OUT_OF_MEMORY;
+    }
+    if (hipMemset(b->d_ctr, 0, sizeof(uint32_t) * need) != hipSuccess) return GFFT_ERR_HIP;
+    b->ctr_words = need;
+  }
+  return GFFT_OK;
+}
+
+gfft::batch::PersistArgs persist_args(const gfft_batch_s* b, int mode) {
+  using namespace gfft::batch;
+  PersistArgs a{};
+  const gfft_batch_s::DevPlan& dp = b->plan[mode];
+  a.xf = b->d_xf;
+  a.seg = dp.seg;
+  a.team = dp.team;
+  a.slot = dp.slot;
+  a.entry = dp.entry;
+  a.nslots = dp.nslots;
+  a.claim_words = dp.claim_words;
+  a.ctr = b->d_ctr;
+  a.host_abort = b->h_abort;
+  const double ticks = b->wait_limit_s * (double)b->wall_khz * 1e3;
+  a.wait_ticks = ticks >= 1.8e19 ? ~0ull : (uint64_t)ticks;
+  a.steal_ticks = (uint64_t)(20e-6 * (double)b->wall_khz * 1e3);  // 20 us
+  for (int c = 0; c < kClasses; ++c)
+    a.tw[c] = static_cast<const gfft::cx<float>*>(b->d_tw[c][kSetVariant[b->pset][c]]);
+  return a;
+}
+
+uint32_t persist_grid(const gfft_batch_s* b, int mode) { return std::max<uint32_t>(1u, b->plan[mode].nslots); }
+
+int launch_persistent(gfft_batch_s* b, int mode, hipStream_t s) {
+  if (b->plan[mode].nslots == 0) return GFFT_OK;
+  const gfft::batch::PersistArgs a = persist_args(b, mode);
+  return gfft::batch::launch_persist(b->pset, persist_grid(b, mode), b->plds, a, s) == hipSuccess ? GFFT_OK
+                                                                                                   : GFFT_ERR_HIP;
+}
+
+int build_persist_graph(gfft_batch_s* b, int mode, hipGraph_t* out) {
+  hipGraph_t g = nullptr;
+  if (hipGraphCreate(&g, 0) != hipSuccess) return GFFT_ERR_HIP;
+  if (b->plan[mode].nslots > 0) {
+    gfft::batch::PersistArgs a = persist_args(b, mode);
+    void* params[] = {&a};
+    hipKernelNodeParams p{};
+    p.func = const_cast<void*>(gfft::batch::persist_kernel(b->pset));
+    p.gridDim = dim3(persist_grid(b, mode));
+    p.blockDim = dim3(gfft::batch::kBT);
+    p.sharedMemBytes = (unsigned)b->plds;
+    p.kernelParams = params;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    if (hipGraphAddKernelNode(&node, g, nullptr, 0, &p) != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return GFFT_ERR_HIP;
+    }
+  }
+  *out = g;
   return GFFT_OK;
 }
 
@@ -457,6 +495,85 @@ int upload_xf(gfft_batch_s* b) {
              : GFFT_ERR_HIP;
 }
 
+// Times `reps` eager executions of `mode` after one warm-up; ms per execution.
+int time_exec(gfft_batch_s* b, int mode, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* ms) {
+  auto launch = [&] {
+    return b->sched == GFFT_BATCH_PERSISTENT ? launch_persistent(b, mode, s) : launch_eager(b, mode, s);
+  };
+  int st = launch();
+  if (st != GFFT_OK) return st;
+  if (hipEventRecord(e0, s) != hipSuccess) return GFFT_ERR_HIP;
+  for (int r = 0; r < reps && st == GFFT_OK; ++r) st = launch();
+  if (st != GFFT_OK) return st;
+  float t = 0;
+  if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+      hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+    return GFFT_ERR_HIP;
+  *ms = t / reps;
+  return GFFT_OK;
+}
+
+// Persistent scheduler: every (variant set, team factor) pair; a candidate
+// must win by 1 %; a candidate whose kernel does not launch is skipped.
+int tune_persistent(gfft_batch_s* b, int mode, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* best) {
+  static const int kTeamFactors[] = {1, 2, 4};
+  int st = GFFT_OK, best_set = -1, best_factor = b->pteam;
+  float best_ms = 1e30f;
+  b->sched = GFFT_BATCH_PERSISTENT;
+  for (int set = 0; set < gfft::batch::kSets && st == GFFT_OK; ++set)
+    for (int f : kTeamFactors) {
+      b->pset = set;
+      b->pteam = f;
+      if ((st = build_plans(b)) != GFFT_OK) break;
+      float ms = 0;
+      const int rt = time_exec(b, mode, reps, s, e0, e1, &ms);
+      if (rt == GFFT_ERR_HIP) {  // this set's kernel cannot run here: skip it
+        (void)hipGetLastError();
+        if (hipDeviceSynchronize() != hipSuccess) return GFFT_ERR_HIP;
+        break;
+      }
+      if ((st = rt) != GFFT_OK) break;
+      if (ms < 0.99f * best_ms) {
+        best_ms = ms;
+        best_set = set;
+        best_factor = f;
+      }
+    }
+  if (st != GFFT_OK) return st;
+  if (best_set < 0) return GFFT_ERR_HIP;
+  b->pset = best_set;
+  b->pteam = best_factor;
+  *best = best_ms;
+  return build_plans(b);
+}
+
+// Level scheduler: every class's variants and the per-transform-plan choice,
+// each class's level chain timed alone.
+int tune_levels(gfft_batch_s* b, int mode, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  int st = GFFT_OK;
+  for (int c = 0; c < gfft::batch::kClasses && st == GFFT_OK; ++c) {
+    if (b->n[c] == 0) continue;
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v <= gfft::batch::kPlainVariant && st == GFFT_OK; ++v) {
+      b->var[c] = v;
+      if ((st = launch_chain(b, c, mode, s)) != GFFT_OK) break;  // warm-up
+      if (hipEventRecord(e0, s) != hipSuccess) st = GFFT_ERR_HIP;
+      for (int r = 0; r < reps && st == GFFT_OK; ++r) st = launch_chain(b, c, mode, s);
+      float ms = 0;
+      if (st == GFFT_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = GFFT_ERR_HIP;
+      if (st == GFFT_OK && ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    b->var[c] = best;
+  }
+  return st;
+}
+
 }  // namespace
 
 extern "C" {
@@ -464,15 +581,16 @@ extern "C" {
 int gfft_batch_query(int64_t nx, int64_t ny, int64_t nz, int prec) {
   if (prec != GFFT_F32) return GFFT_ERR_UNSUPPORTED_SIZE;
   if (nx < 1 || ny < 1 || nz < 1) return GFFT_ERR_INVALID_DIMS;
-  const int c = gfft::class_of(nz);
+  const int c = gfft::batch::class_of(nz);
   if (c < 0) return GFFT_ERR_UNSUPPORTED_SIZE;
-  if (ny > 1 && gfft::class_of(ny) != c) return GFFT_ERR_UNSUPPORTED_SIZE;
-  if (nx > 1 && gfft::class_of(nx) != c) return GFFT_ERR_UNSUPPORTED_SIZE;
+  if (ny > 1 && gfft::batch::class_of(ny) != c) return GFFT_ERR_UNSUPPORTED_SIZE;
+  if (nx > 1 && gfft::batch::class_of(nx) != c) return GFFT_ERR_UNSUPPORTED_SIZE;
   return GFFT_OK;
 }
 
 int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec) {
   using namespace gfft;
+  using namespace gfft::batch;
   if (!out || count < 1 || !dims) return GFFT_ERR_INVALID_ARG;
   *out = nullptr;
   for (int i = 0; i < count; ++i) {
@@ -603,6 +721,19 @@ int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec)
       hipMemcpy(b->d_prefix, prefix.data(), sizeof(uint32_t) * prefix.size(), hipMemcpyHostToDevice) != hipSuccess)
     st = GFFT_ERR_HIP;
   if (st == GFFT_OK) st = upload_xf(b);
+  // persistent execution: the host abort word, plans and counters
+  if (st == GFFT_OK) {
+    if (hipHostMalloc((void**)&b->h_abort, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+      b->h_abort = nullptr;
+      st = GFFT_ERR_OUT_OF_MEMORY;
+    }
+    if (st == GFFT_OK) *b->h_abort = 0;
+    if (st == GFFT_OK &&
+        (hipDeviceGetAttribute(&b->wall_khz, hipDeviceAttributeWallClockRate, b->device) != hipSuccess ||
+         b->wall_khz <= 0))
+      st = GFFT_ERR_HIP;
+    if (st == GFFT_OK) st = build_plans(b);
+  }
   if (st != GFFT_OK) {
     release(b);
     delete b;
@@ -618,7 +749,7 @@ int gfft_batch_set_data(gfft_batch b, void* const* ptrs) {
     if (!ptrs[i] || (reinterpret_cast<uintptr_t>(ptrs[i]) & 7u) != 0) return GFFT_ERR_INVALID_ARG;
   for (int i = 0; i < b->count; ++i) b->hx[b->pos[i]].data = static_cast<float*>(ptrs[i]);
   const int st = upload_xf(b);  // synchronises the device first
-  if (any_plain(b)) drop_graphs(b);
+  if (b->sched == GFFT_BATCH_LEVELS && any_plain(b)) drop_graphs(b);
   if (st == GFFT_OK) b->data_set = true;
   return st;
 }
@@ -630,7 +761,7 @@ int gfft_batch_set_scales(gfft_batch b, const double* r2c_scale, const double* c
     b->hx[b->pos[i]].scale_c2r = c2r_scale ? (float)c2r_scale[i] : 1.0f;
   }
   const int st = upload_xf(b);
-  if (any_plain(b)) drop_graphs(b);
+  if (b->sched == GFFT_BATCH_LEVELS && any_plain(b)) drop_graphs(b);
   return st;
 }
 
@@ -638,6 +769,7 @@ int gfft_batch_exec(gfft_batch b, int mode, gfft_stream_t stream) {
   if (!b || mode < 0 || mode > 2 || !b->data_set) return GFFT_ERR_INVALID_ARG;
   const int st = check_dev(b);
   if (st != GFFT_OK) return st;
+  if (b->sched == GFFT_BATCH_PERSISTENT) return launch_persistent(b, mode, reinterpret_cast<hipStream_t>(stream));
   return launch_eager(b, mode, reinterpret_cast<hipStream_t>(stream));
 }
 
@@ -647,7 +779,7 @@ int gfft_batch_exec_graph(gfft_batch b, int mode, gfft_stream_t stream) {
   if (st != GFFT_OK) return st;
   if (!b->gexec[mode]) {
     hipGraph_t g = nullptr;
-    st = build_graph(b, mode, &g);
+    st = b->sched == GFFT_BATCH_PERSISTENT ? build_persist_graph(b, mode, &g) : build_graph(b, mode, &g);
     if (st != GFFT_OK) return st;
     hipGraphExec_t ge = nullptr;
     if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
@@ -662,8 +794,8 @@ int gfft_batch_exec_graph(gfft_batch b, int mode, gfft_stream_t stream) {
 }
 
 int gfft_batch_set_variant(gfft_batch b, int64_t edge, int variant) {
-  if (!b || variant < 0 || variant > gfft::kPlainVariant) return GFFT_ERR_INVALID_ARG;
-  const int c = gfft::class_of(edge);
+  if (!b || variant < 0 || variant > gfft::batch::kPlainVariant) return GFFT_ERR_INVALID_ARG;
+  const int c = gfft::batch::class_of(edge);
   if (c < 0) return GFFT_ERR_INVALID_ARG;
   if (hipDeviceSynchronize() != hipSuccess) return GFFT_ERR_HIP;
   b->var[c] = variant;
@@ -673,7 +805,7 @@ int gfft_batch_set_variant(gfft_batch b, int64_t edge, int variant) {
 
 int gfft_batch_variant(gfft_batch b, int64_t edge, int* variant) {
   if (!b || !variant) return GFFT_ERR_INVALID_ARG;
-  const int c = gfft::class_of(edge);
+  const int c = gfft::batch::class_of(edge);
   if (c < 0) return GFFT_ERR_INVALID_ARG;
   *variant = b->var[c];
   return GFFT_OK;
@@ -691,26 +823,18 @@ int gfft_batch_tune(gfft_batch b, int mode, int reps, gfft_stream_t stream) {
     (void)hipEventDestroy(e0);
     return GFFT_ERR_HIP;
   }
-  for (int c = 0; c < gfft::kClasses && st == GFFT_OK; ++c) {
-    if (b->n[c] == 0) continue;
-    int best = 0;
-    float best_ms = 1e30f;
-    for (int v = 0; v <= gfft::kPlainVariant && st == GFFT_OK; ++v) {
-      b->var[c] = v;
-      if ((st = launch_chain(b, c, mode, s)) != GFFT_OK) break;  // warm-up
-      if (hipEventRecord(e0, s) != hipSuccess) st = GFFT_ERR_HIP;
-      for (int r = 0; r < reps && st == GFFT_OK; ++r) st = launch_chain(b, c, mode, s);
-      float ms = 0;
-      if (st == GFFT_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-        st = GFFT_ERR_HIP;
-      if (st == GFFT_OK && ms < best_ms) {
-        best_ms = ms;
-        best = v;
-      }
-    }
-    b->var[c] = best;
+  drop_graphs(b);
+  // both schedulers tuned, then the faster kept (eager timings of `mode`)
+  float ms_p = 1e30f, ms_l = 1e30f;
+  st = tune_persistent(b, mode, reps, s, e0, e1, &ms_p);
+  if (st == GFFT_OK) {
+    b->sched = GFFT_BATCH_LEVELS;
+    st = tune_levels(b, mode, reps, s, e0, e1);
   }
+  if (st == GFFT_OK) st = time_exec(b, mode, reps, s, e0, e1, &ms_l);
+  if (st == GFFT_OK) b->sched = ms_p <= ms_l ? GFFT_BATCH_PERSISTENT : GFFT_BATCH_LEVELS;
+  b->tuned_ms[0] = ms_p;
+  b->tuned_ms[1] = ms_l;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   drop_graphs(b);
@@ -725,15 +849,92 @@ int gfft_batch_data_bytes(gfft_batch b, int index, size_t* bytes) {
 
 int gfft_batch_tiles(gfft_batch b, int mode, uint32_t* tiles, uint32_t* launches) {
   if (!b || !tiles || mode < 0 || mode > 2) return GFFT_ERR_INVALID_ARG;
+  if (b->sched == GFFT_BATCH_PERSISTENT) {
+    *tiles = b->plan[mode].tiles;
+    if (launches) *launches = b->plan[mode].nslots ? 1u : 0u;
+    return GFFT_OK;
+  }
   uint32_t t = 0, n = 0;
-  for (int c = 0; c < gfft::kClasses; ++c)
-    for (int l = 0; l < kModeLevels[mode] && b->var[c] != gfft::kPlainVariant; ++l) {
+  for (int c = 0; c < gfft::batch::kClasses; ++c)
+    for (int l = 0; l < kModeLevels[mode] && b->var[c] != gfft::batch::kPlainVariant; ++l) {
       const uint32_t k = b->level_tiles[c][b->var[c]][mode][l];
       t += k;
       n += k ? 1u : 0u;
     }
   *tiles = t;
   if (launches) *launches = n;
+  return GFFT_OK;
+}
+
+int gfft_batch_set_scheduler(gfft_batch b, int scheduler) {
+  if (!b || (scheduler != GFFT_BATCH_PERSISTENT && scheduler != GFFT_BATCH_LEVELS)) return GFFT_ERR_INVALID_ARG;
+  if (scheduler != b->sched) {
+    drop_graphs(b);
+    b->sched = scheduler;
+  }
+  return GFFT_OK;
+}
+
+int gfft_batch_scheduler(gfft_batch b, int* scheduler) {
+  if (!b || !scheduler) return GFFT_ERR_INVALID_ARG;
+  *scheduler = b->sched;
+  return GFFT_OK;
+}
+
+int gfft_batch_set_persist(gfft_batch b, int set, int team_factor) {
+  if (!b || set < 0 || set >= gfft::batch::kSets || team_factor < 1 || team_factor > 64) return GFFT_ERR_INVALID_ARG;
+  drop_graphs(b);
+  b->pset = set;
+  b->pteam = team_factor;
+  return build_plans(b);
+}
+
+int gfft_batch_persist(gfft_batch b, int* set, int* team_factor, int* variants) {
+  if (!b || !set || !team_factor) return GFFT_ERR_INVALID_ARG;
+  *set = b->pset;
+  *team_factor = b->pteam;
+  if (variants)
+    for (int c = 0; c < gfft::batch::kClasses; ++c) variants[c] = gfft::batch::kSetVariant[b->pset][c];
+  return GFFT_OK;
+}
+
+int gfft_batch_tuned_ms(gfft_batch b, float* persistent_ms, float* levels_ms) {
+  if (!b || !persistent_ms || !levels_ms) return GFFT_ERR_INVALID_ARG;
+  *persistent_ms = b->tuned_ms[0];
+  *levels_ms = b->tuned_ms[1];
+  return GFFT_OK;
+}
+
+int gfft_batch_plan_info(gfft_batch b, int mode, uint32_t* slots, uint32_t* teams, uint32_t* members) {
+  if (!b || mode < 0 || mode > 2 || !slots || !teams || !members) return GFFT_ERR_INVALID_ARG;
+  *slots = b->plan[mode].nslots;
+  *teams = b->plan[mode].teams_k2;
+  *members = b->plan[mode].members;
+  return GFFT_OK;
+}
+
+int gfft_batch_set_wait_limit(gfft_batch b, double seconds) {
+  if (!b || !(seconds >= 0.0)) return GFFT_ERR_INVALID_ARG;
+  drop_graphs(b);  // the bound is a kernel argument of the captured launch
+  b->wait_limit_s = seconds;
+  return GFFT_OK;
+}
+
+int gfft_batch_set_abort(gfft_batch b, int abort) {
+  if (!b) return GFFT_ERR_INVALID_ARG;
+  __atomic_store_n(b->h_abort, abort ? 1u : 0u, __ATOMIC_SEQ_CST);
+  return GFFT_OK;
+}
+
+int gfft_batch_status(gfft_batch b, int* status) {
+  if (!b || !status) return GFFT_ERR_INVALID_ARG;
+  if (hipDeviceSynchronize() != hipSuccess) return GFFT_ERR_HIP;
+  uint32_t w = 0;
+  if (hipMemcpy(&w, b->d_ctr + gfft::kCtrStatus, sizeof w, hipMemcpyDeviceToHost) != hipSuccess) return GFFT_ERR_HIP;
+  const uint32_t zero = 0;
+  if (w && hipMemcpy(b->d_ctr + gfft::kCtrStatus, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess)
+    return GFFT_ERR_HIP;
+  *status = (w & gfft::kFailAborted) ? GFFT_ERR_ABORTED : (w & gfft::kFailTimeout) ? GFFT_ERR_TIMEOUT : GFFT_OK;
   return GFFT_OK;
 }
 

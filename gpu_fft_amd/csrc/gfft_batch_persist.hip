@@ -1,0 +1,355 @@
+// Persistent batched kernel: every axis pass of every transform of a batch
+// mode in one launch, eager or captured in a graph, following the plan of
+// gfft_batch_sched.h (one workgroup per slot; solo transforms run whole by
+// one workgroup, larger ones by a team whose members own interleaved tile
+// shares of each level).
+//
+// Solo transforms need no synchronisation beyond the workgroup barrier
+// between levels (one workgroup's stores are visible to its own later loads;
+// its data can stay in the L2 of its XCD across the six passes).
+//
+// Teams synchronise per level through share state words (free, claimed,
+// done), also across the XCDs' separate L2 caches:
+//   run a share: claim it (compare-and-swap, agent scope), run its tiles,
+//             every thread an agent-scope release fence (L2 write-back of
+//             its stores), workgroup barrier, one agent-scope release store
+//             of "done" to the share's word;
+//   wait:     the first wavefront reads the level's share words (agent-scope
+//             atomic loads, 64 per instruction) until all are done, then an
+//             agent-scope acquire fence; barrier; every thread acquires too.
+// While waiting, after steal_ticks, a member claims and runs any share still
+// free (its owner has not started), so a team never depends on all of its
+// members being resident.  Every wait is bounded: it gives up when the
+// launch's wall-clock bound passes or when the host sets the batch's abort
+// word (pinned host memory, system-scope loads), and marks the launch failed;
+// a failed launch's workgroups stop computing and leave.  The last workgroup
+// to leave moves the failure bits into the sticky status word and zeroes the
+// other counter words, so the next launch (or graph replay) starts from zero
+// without a memset.
+#include "gfft_batch_tiles.h"
+
+namespace gfft {
+namespace batch {
+namespace {
+
+template <int S, int C>
+using SCfg = BCfg<C, kSetVariant[S][C]>;
+
+// Tiles first, first + step, ... (count of them) of one level of one
+// transform, software-pipelined: while one tile runs its LDS stages, the
+// global loads of the next are in flight in a second register set (stage
+// barriers wait for LDS traffic only; the twiddles are copied to LDS, after
+// the tile area, so no global load in the stages waits behind the prefetch).
+// Tiles of a level are disjoint, so loading the next before storing the
+// current is safe in place.  The thread's lane indexes are made opaque at
+// the top of every tile, so the per-element offsets are recomputed per tile
+// instead of being held in registers across the loop (which doubled the
+// register count).
+template <typename F>
+struct Seq {
+  using SC = typename F::SC;
+  static constexpr int P = F::P, W = F::W, N = SC::N;
+  using R2C = RowsR2C<float, SC, P, W, true>;
+  using C2R = RowsC2R<float, SC, P, W, true>;
+  template <bool INV>
+  using COLS = ColsPass<float, INV, SC, P, W, true>;
+  static constexpr int kTw = twiddle_count<SC>();
+  static constexpr int lds_entries() { return F::lds_entries() + kTw; }
+
+  __device__ static void rows_r2c(const BXform& x, uint32_t first, uint32_t step, uint32_t count,
+                                  const cx<float>* stw, cx<float>* smem) {
+    RowGeom g;
+    g.nrows = x.nrows;
+    g.K = x.K;
+    g.scale = 1.0;
+    const int gi0 = threadIdx.x / R2C::TPT, t0 = threadIdx.x % R2C::TPT;
+    cx<float> cur[R2C::E::PV], nxt[R2C::E::PV];
+    R2C::load(nxt, x.data, g, (int64_t)first * W + gi0, t0);
+    for (uint32_t k = 0; k < count; ++k) {
+      int gi = gi0, t = t0;
+      __asm__ volatile("" : "+v"(gi), "+v"(t));
+      RowLds<float, N> L{smem + gi * R2C::LROW};
+#pragma unroll
+      for (int i = 0; i < R2C::E::PV; ++i) cur[i] = nxt[i];
+      if (k + 1 < count) R2C::load(nxt, x.data, g, (int64_t)(first + (k + 1) * step) * W + gi, t);
+      R2C::E::run(cur, t, L, stw);
+      R2C::finish(cur, x.data, g, (int64_t)(first + k * step) * W + gi, t, L);
+    }
+  }
+
+  __device__ static void rows_c2r(const BXform& x, uint32_t first, uint32_t step, uint32_t count,
+                                  const cx<float>* stw, cx<float>* smem) {
+    RowGeom g;
+    g.nrows = x.nrows;
+    g.K = x.K;
+    g.scale = x.scale_c2r;
+    const int gi0 = threadIdx.x / C2R::TPT, t0 = threadIdx.x % C2R::TPT;
+    cx<float> ca[C2R::NI], cb[C2R::NI], na[C2R::NI], nb[C2R::NI], v[C2R::E::PV];
+    C2R::load(na, nb, x.data, g, (int64_t)first * W + gi0, t0);
+    for (uint32_t k = 0; k < count; ++k) {
+      int gi = gi0, t = t0;
+      __asm__ volatile("" : "+v"(gi), "+v"(t));
+      cx<float>* S = smem + gi * C2R::LROW;
+      RowLds<float, N> L{S};
+#pragma unroll
+      for (int i = 0; i < C2R::NI; ++i) {
+        ca[i] = na[i];
+        cb[i] = nb[i];
+      }
+      if (k + 1 < count) C2R::load(na, nb, x.data, g, (int64_t)(first + (k + 1) * step) * W + gi, t);
+      stage_barrier<C2R::BM>();  // the previous tile's stage reads of S are done
+      C2R::build(v, ca, cb, S, t);
+      C2R::E::run(v, t, L, stw);
+      C2R::store(v, x.data, g, (int64_t)(first + k * step) * W + gi, t);
+    }
+  }
+
+  template <bool INV>
+  __device__ static void cols(const BXform& x, int axis, uint32_t first, uint32_t step, uint32_t count,
+                              const cx<float>* stw, cx<float>* smem) {
+    using K_ = COLS<INV>;
+    ColGeom g;
+    if (axis == 1) {
+      g.ncols = x.ncols_y;
+      g.in = g.out = dmap((uint32_t)x.K, (int64_t)x.ny * x.K, x.K);
+    } else {
+      g.ncols = x.ncols_x;
+      g.in = g.out = dmap(x.ncols_x, 0, (int64_t)x.ncols_x);
+    }
+    // the forward scale belongs to the last R2C pass (x, or y when x is length 1)
+    const bool last_fwd = !INV && (axis == 2 || (axis == 1 && x.cls[2] < 0));
+    g.scale = last_fwd ? x.scale_r2c : 1.0;
+    cx<float>* d = reinterpret_cast<cx<float>*>(x.data);
+    const int c0 = threadIdx.x % W, t0 = threadIdx.x / W;
+    cx<float> cur[K_::E::PV], nxt[K_::E::PV];
+    K_::load(nxt, d, g, first, c0, t0);
+    for (uint32_t k = 0; k < count; ++k) {
+      int c = c0, t = t0;
+      __asm__ volatile("" : "+v"(c), "+v"(t));
+      ColLds<float, W> L{smem, c};
+#pragma unroll
+      for (int i = 0; i < K_::E::PV; ++i) cur[i] = nxt[i];
+      if (k + 1 < count) K_::load(nxt, d, g, first + (k + 1) * step, c, t);
+      K_::E::run(cur, t, L, stw);
+      K_::store(cur, d, g, first + k * step, c, t);
+    }
+  }
+
+  __device__ static void run(const BXform& x, int axis, int inv, uint32_t first, uint32_t step, uint32_t count,
+                             const cx<float>* tw, cx<float>* smem) {
+    cx<float>* stw = smem + F::lds_entries();
+    for (int i = threadIdx.x; i < SC::twoff(SC::S); i += kBT) stw[i] = tw[i];
+    __syncthreads();  // the table is complete (and the previous call's LDS reads are done)
+    if (count > 0) {
+      if (axis == 0) {
+        if (inv)
+          rows_c2r(x, first, step, count, stw, smem);
+        else
+          rows_r2c(x, first, step, count, stw, smem);
+      } else if (inv) {
+        cols<true>(x, axis, first, step, count, stw, smem);
+      } else {
+        cols<false>(x, axis, first, step, count, stw, smem);
+      }
+    }
+    __syncthreads();  // every LDS access of this call is done before the next call's writes
+  }
+};
+
+// One class's tile code as a function of its own (not inlined): its
+// registers are allocated for it alone, so the kernel needs the largest
+// class's count rather than what one function holding all eight would.
+template <int S, int C>
+__device__ __attribute__((noinline)) void class_tiles(const BXform& x, int axis, int inv, uint32_t first,
+                                                      uint32_t step, uint32_t count, const cx<float>* tw,
+                                                      cx<float>* smem) {
+  Seq<SCfg<S, C>>::run(x, axis, inv, first, step, count, tw, smem);
+}
+
+template <int S>
+__device__ __forceinline__ void run_set_tiles(const BXform& x, const PSeg& sg, uint32_t first, uint32_t step,
+                                              uint32_t count, const PersistArgs& a, cx<float>* smem) {
+  switch (sg.cls) {
+    case 0: class_tiles<S, 0>(x, sg.axis, sg.inv, first, step, count, a.tw[0], smem); break;
+    case 1: class_tiles<S, 1>(x, sg.axis, sg.inv, first, step, count, a.tw[1], smem); break;
+    case 2: class_tiles<S, 2>(x, sg.axis, sg.inv, first, step, count, a.tw[2], smem); break;
+    case 3: class_tiles<S, 3>(x, sg.axis, sg.inv, first, step, count, a.tw[3], smem); break;
+    case 4: class_tiles<S, 4>(x, sg.axis, sg.inv, first, step, count, a.tw[4], smem); break;
+    case 5: class_tiles<S, 5>(x, sg.axis, sg.inv, first, step, count, a.tw[5], smem); break;
+    case 6: class_tiles<S, 6>(x, sg.axis, sg.inv, first, step, count, a.tw[6], smem); break;
+    default: class_tiles<S, 7>(x, sg.axis, sg.inv, first, step, count, a.tw[7], smem); break;
+  }
+}
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool launch_failed(const PersistArgs& a) { return ld_agent(a.ctr + kCtrFail) != 0u; }
+
+// tiles m, m + k, ... of level sg, then publish the share as done
+template <int S>
+__device__ void run_share(const PersistArgs& a, const BXform& x, const PSeg& sg, uint32_t k, uint32_t m,
+                          cx<float>* smem) {
+  run_set_tiles<S>(x, sg, m, k, share_tiles(sg.ntiles, k, m), a, smem);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this thread's stores, written back
+  __syncthreads();                                    // every thread's
+  if (threadIdx.x == 0)
+    __hip_atomic_store(a.ctr + kCtrShares + sg.claim0 + m, kShareDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool claim_share(const PersistArgs& a, const PSeg& sg, uint32_t m) {
+  uint32_t expect = kShareFree;
+  return __hip_atomic_compare_exchange_strong(a.ctr + kCtrShares + sg.claim0 + m, &expect, kShareClaimed,
+                                              __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait-step outcomes, decided by the first wavefront
+constexpr uint32_t kStepDone = 0xffffffffu, kStepFail = 0xfffffffeu, kStepSpin = 0xfffffffdu;
+
+// First wavefront only (all 64 lanes): one look at the level's share words.
+// Returns kStepDone, kStepFail, kStepSpin, or the index (< k) of a share
+// this workgroup has just claimed (steal).  Every decision is made by lane 0
+// and broadcast, so the wavefront's control flow stays uniform.
+__device__ uint32_t wait_step(const PersistArgs& a, const PSeg& sg, uint32_t k, uint64_t t0, uint32_t* next) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t* w = a.ctr + kCtrShares + sg.claim0;
+  bool mine_done = true, mine_free = false;
+  for (uint32_t j = lane; j < k; j += 64) {
+    const uint32_t v = ld_agent(w + j);
+    mine_done = mine_done && v == kShareDone;
+    mine_free = mine_free || v == kShareFree;
+  }
+  if (__all(mine_done)) return kStepDone;
+  const bool any_free = __ballot(mine_free) != 0ull;
+  uint32_t out = kStepSpin;
+  if (lane == 0) {
+    const uint64_t el = wall_clock64() - t0;
+    if (launch_failed(a)) {
+      out = kStepFail;
+    } else if (a.host_abort &&
+               __hip_atomic_load(a.host_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+      __hip_atomic_fetch_or(a.ctr + kCtrFail, kFailAborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      out = kStepFail;
+    } else if (el > a.wait_ticks) {
+      __hip_atomic_fetch_or(a.ctr + kCtrFail, kFailTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      out = kStepFail;
+    } else if (any_free && el > a.steal_ticks) {
+      // steal the first free share at or after the rotating start *next*
+      for (uint32_t n = 0; n < k; ++n) {
+        uint32_t j = *next + n;
+        if (j >= k) j -= k;
+        if (ld_agent(w + j) != kShareFree) continue;
+        uint32_t expect = kShareFree;
+        if (__hip_atomic_compare_exchange_strong(a.ctr + kCtrShares + sg.claim0 + j, &expect, kShareClaimed,
+                                                 __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          out = j;
+          *next = j + 1 < k ? j + 1 : 0u;
+          break;
+        }
+      }
+    }
+  }
+  out = __shfl(out, 0);
+  if (out == kStepSpin) __builtin_amdgcn_s_sleep(2);
+  return out;
+}
+
+template <int S>
+__global__ __launch_bounds__(kBT) void k_batch_persist(PersistArgs a) {
+  extern __shared__ cx<float> smem[];
+  __shared__ uint32_t s_step, s_last, s_claimed;
+  const uint32_t tid = threadIdx.x;
+  bool failed = false;
+  if (blockIdx.x < a.nslots) {
+    const PSlot sl = a.slot[blockIdx.x];
+    for (uint32_t i = 0; i < sl.count && !failed; ++i) {
+      const PEntry e = a.entry[sl.first + i];
+      const PTeam tm = a.team[e.team];
+      const BXform x = a.xf[e.team];
+      for (uint32_t s = tm.seg0; s < tm.seg0 + tm.nseg && !failed; ++s) {
+        const PSeg sg = a.seg[s];
+        if (tm.k == 1) {  // solo: the whole level here, then on to the next
+          run_set_tiles<S>(x, sg, 0, 1, sg.ntiles, a, smem);
+          continue;
+        }
+        // team member: the own share, then wait (stealing) for the level
+        if (tid == 0) s_claimed = claim_share(a, sg, e.member) ? 1u : 0u;
+        __syncthreads();
+        if (s_claimed) run_share<S>(a, x, sg, tm.k, e.member, smem);
+        uint64_t t0 = 0;
+        uint32_t next = e.member + 1 < tm.k ? e.member + 1 : 0u;
+        if (tid < 64) t0 = wall_clock64();
+        for (;;) {
+          if (tid < 64) {
+            const uint32_t st = wait_step(a, sg, tm.k, t0, &next);
+            if (tid == 0) {
+              if (st == kStepDone) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+              s_step = st;
+            }
+          }
+          __syncthreads();
+          const uint32_t st = s_step;
+          __syncthreads();  // s_step is read before the next step writes it
+          if (st == kStepDone) break;
+          if (st == kStepFail) {
+            failed = true;
+            break;
+          }
+          if (st < tm.k) run_share<S>(a, x, sg, tm.k, st, smem);  // a stolen share
+        }
+        if (!failed) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every thread: the level's stores
+      }
+    }
+  }
+  // leave; the last workgroup out resets the counters for the next launch
+  if (tid == 0) {
+    const uint32_t prev = __hip_atomic_fetch_add(a.ctr + kCtrExited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = prev == gridDim.x - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (s_last) {
+    if (tid == 0) {
+      const uint32_t f = ld_agent(a.ctr + kCtrFail);
+      if (f) __hip_atomic_fetch_or(a.ctr + kCtrStatus, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t words = kCtrShares + a.claim_words;
+    for (uint32_t i = tid; i < words; i += kBT)
+      if (i != kCtrStatus) __hip_atomic_store(a.ctr + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <int S>
+size_t lds_of_set(const bool* present) {
+  const int e[kClasses] = {Seq<SCfg<S, 0>>::lds_entries(), Seq<SCfg<S, 1>>::lds_entries(),
+                           Seq<SCfg<S, 2>>::lds_entries(), Seq<SCfg<S, 3>>::lds_entries(),
+                           Seq<SCfg<S, 4>>::lds_entries(), Seq<SCfg<S, 5>>::lds_entries(),
+                           Seq<SCfg<S, 6>>::lds_entries(), Seq<SCfg<S, 7>>::lds_entries()};
+  int m = 1;
+  for (int c = 0; c < kClasses; ++c)
+    if (present[c] && e[c] > m) m = e[c];
+  return (size_t)m * sizeof(cx<float>);
+}
+
+}  // namespace
+
+const void* persist_kernel(int set) {
+  return set == 0 ? reinterpret_cast<const void*>(&k_batch_persist<0>)
+                  : reinterpret_cast<const void*>(&k_batch_persist<1>);
+}
+
+size_t persist_lds_bytes(int set, const bool* present) {
+  return set == 0 ? lds_of_set<0>(present) : lds_of_set<1>(present);
+}
+
+hipError_t launch_persist(int set, uint32_t grid, size_t lds, const PersistArgs& a, hipStream_t s) {
+  if (set == 0)
+    hipLaunchKernelGGL((k_batch_persist<0>), dim3(grid), dim3(kBT), lds, s, a);
+  else
+    hipLaunchKernelGGL((k_batch_persist<1>), dim3(grid), dim3(kBT), lds, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace batch
+}  // namespace gfft

@@ -1,0 +1,186 @@
+// Tile code shared by the batched kernels (gfft_batch.hip: one launch per
+// size class and level; gfft_batch_persist.hip: one persistent launch for
+// the whole batch).  Transforms are grouped by size class (all three edges
+// of a transform belong to one class); each class has kVariants compiled
+// variants (radix order; points per thread, which sets the pencils per
+// 256-thread tile) and every tile runs the same register-engine tile code as
+// the single-transform kernels.
+#pragma once
+#include "gfft_batch_sched.h"
+#include "gfft_kernels.h"
+
+namespace gfft {
+namespace batch {
+
+constexpr int kBT = 256;  // threads per workgroup, every tile kind
+constexpr int kClasses = 8;
+constexpr int kVariants = 3;
+constexpr int kPlainVariant = kVariants;  // per-transform plans
+constexpr int kMaxLevels = 6;
+constexpr int kLanes = 4;  // internal streams of the eager path
+
+// Variant V of size class C: radix order and points per thread P; with
+// TPT = N/P threads per transform a tile carries W = 256/TPT row pairs (z)
+// or columns (y, x).
+template <int C, int V>
+struct BVar;
+#define GFFT_BVAR(C, V, P_, ...)     \
+  template <>                        \
+  struct BVar<C, V> {                \
+    using SC = Sched<__VA_ARGS__>;   \
+    static constexpr int P = P_;     \
+  };
+GFFT_BVAR(0, 0, 8, 8, 4)
+GFFT_BVAR(0, 1, 8, 4, 8)
+GFFT_BVAR(0, 2, 4, 8, 4)
+GFFT_BVAR(1, 0, 12, 4, 4, 3)
+GFFT_BVAR(1, 1, 12, 3, 4, 4)
+GFFT_BVAR(1, 2, 6, 4, 4, 3)
+GFFT_BVAR(2, 0, 8, 8, 8)
+GFFT_BVAR(2, 1, 4, 4, 4, 4)
+GFFT_BVAR(2, 2, 16, 8, 8)
+GFFT_BVAR(3, 0, 24, 8, 4, 3)
+GFFT_BVAR(3, 1, 12, 8, 4, 3)
+GFFT_BVAR(3, 2, 24, 3, 8, 4)
+GFFT_BVAR(4, 0, 8, 8, 4, 4)
+GFFT_BVAR(4, 1, 8, 4, 4, 8)
+GFFT_BVAR(4, 2, 16, 8, 4, 4)
+GFFT_BVAR(5, 0, 20, 4, 4, 2, 5)
+GFFT_BVAR(5, 1, 10, 8, 4, 5)
+GFFT_BVAR(5, 2, 20, 8, 4, 5)
+GFFT_BVAR(6, 0, 24, 8, 8, 3)
+GFFT_BVAR(6, 1, 12, 8, 8, 3)
+GFFT_BVAR(6, 2, 24, 3, 8, 8)
+GFFT_BVAR(7, 0, 8, 8, 8, 4)
+GFFT_BVAR(7, 1, 16, 8, 8, 4)
+GFFT_BVAR(7, 2, 8, 4, 8, 8)
+#undef GFFT_BVAR
+
+constexpr int kClassN[kClasses] = {32, 48, 64, 96, 128, 160, 192, 256};
+
+template <int C, int V>
+struct BCfg {
+  using SC = typename BVar<C, V>::SC;
+  static constexpr int N = SC::N;
+  static_assert(N == kClassN[C], "variant size");
+  static constexpr int P = BVar<C, V>::P;
+  static constexpr int TPT = N / P;
+  static constexpr int W = kBT / TPT;  // pairs per rows tile = columns per cols tile
+  static_assert(W * TPT == kBT, "tile must use all 256 threads");
+  using R2C = RowsR2C<float, SC, P, W>;
+  using C2R = RowsC2R<float, SC, P, W>;
+  using FWD = ColsPass<float, false, SC, P, W>;
+  using INV = ColsPass<float, true, SC, P, W>;
+  static constexpr int lds_entries() {
+    const int rows = W * R2C::LROW;
+    const int cols = N * W;
+    return rows > cols ? rows : cols;
+  }
+};
+
+inline int class_of(int64_t n) {
+  for (int c = 0; c < kClasses; ++c)
+    if (kClassN[c] == n) return c;
+  return -1;
+}
+
+// One transform of the batch.
+struct BXform {
+  float* data;       // in-place array (Nx, Ny, 2K)
+  int32_t cls[3];    // size class of the z, y, x pass (-1: length-1 strided axis)
+  int32_t K;
+  int32_t ny;
+  int64_t nrows;     // Nx * Ny
+  uint32_t ncols_y;  // Nx * K
+  uint32_t ncols_x;  // Ny * K
+  float scale_r2c, scale_c2r;
+};
+
+// One level launch: one axis pass over every transform of one class.
+struct LevelArgs {
+  const BXform* xf;        // the class's transforms
+  const uint32_t* prefix;  // [count + 1] tile prefix sums of this level
+  const cx<float>* tw;     // twiddle table of the class's variant
+  int32_t count;
+  int32_t axis;            // 0 = z rows, 1 = y columns, 2 = x columns
+  int32_t inv;
+  uint32_t total;          // tiles of this level = prefix[count]
+};
+
+__device__ __forceinline__ ColMap dmap(uint32_t inner, int64_t outer, int64_t rlo) {
+  ColMap m;
+  m.inner = inner;
+  m.nsub = 0x7fffffff;
+  m.outer = outer;
+  m.rlo = rlo;
+  m.rhi = 0;
+  return m;
+}
+
+template <typename F>
+__device__ __forceinline__ void run_tile(const BXform& x, int axis, bool inv, uint32_t tile,
+                                         const cx<float>* tw, cx<float>* smem) {
+  if (axis == 0) {
+    RowGeom g;
+    g.nrows = x.nrows;
+    g.K = x.K;
+    g.scale = inv ? x.scale_c2r : 1.0;
+    if (inv)
+      F::C2R::tile(x.data, g, tw, (int64_t)tile * F::W, smem);
+    else
+      F::R2C::tile(x.data, g, tw, (int64_t)tile * F::W, smem);
+    return;
+  }
+  ColGeom g;
+  if (axis == 1) {
+    g.ncols = x.ncols_y;
+    g.in = g.out = dmap((uint32_t)x.K, (int64_t)x.ny * x.K, x.K);
+  } else {
+    g.ncols = x.ncols_x;
+    g.in = g.out = dmap(x.ncols_x, 0, (int64_t)x.ncols_x);
+  }
+  // the forward scale belongs to the last R2C pass (x, or y/z when x is length 1)
+  const bool last_fwd = !inv && (axis == 2 || (axis == 1 && x.cls[2] < 0));
+  g.scale = last_fwd ? x.scale_r2c : 1.0;
+  cx<float>* d = reinterpret_cast<cx<float>*>(x.data);
+  if (inv)
+    F::INV::tile(d, d, g, tw, tile, smem);
+  else
+    F::FWD::tile(d, d, g, tw, tile, smem);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent execution (gfft_batch_persist.hip): one launch runs the plan of
+// a mode (gfft_batch_sched.h), one workgroup per slot.  A variant set fixes
+// one compiled variant per class; the set's kernel calls the tile code of
+// those eight variants (its register count is the largest of them).
+// ---------------------------------------------------------------------------
+constexpr int kSets = 2;
+// set 0: per class the variant with the fewest registers (kernel: 180
+// VGPRs); set 1: variant 1 of every class (228).  All variants 0 would need
+// 280 registers (with AGPRs), which does not launch.
+constexpr int kSetVariant[kSets][kClasses] = {{2, 2, 1, 1, 0, 1, 1, 0}, {1, 1, 1, 1, 1, 1, 1, 1}};
+
+struct PersistArgs {
+  const BXform* xf;            // descriptors, batch order (= team order)
+  const PSeg* seg;             // levels of every transform
+  const PTeam* team;           // one per transform
+  const PSlot* slot;           // one per workgroup
+  const PEntry* entry;         // what the slots run
+  uint32_t nslots;
+  uint32_t claim_words;
+  uint32_t* ctr;               // counter words (kCtrShares + claim_words)
+  const uint32_t* host_abort;  // pinned host word: nonzero = every wait gives up
+  uint64_t wait_ticks;         // bound of one wait, device wall-clock ticks
+  uint64_t steal_ticks;        // a waiting member steals unclaimed shares after this long
+  const cx<float>* tw[kClasses];  // twiddles of each class's variant in the set
+};
+
+// Host side of the persistent kernels.
+const void* persist_kernel(int set);
+// dynamic LDS bytes of set `set` for the classes marked present
+size_t persist_lds_bytes(int set, const bool* present);
+hipError_t launch_persist(int set, uint32_t grid, size_t lds, const PersistArgs& a, hipStream_t s);
+
+}  // namespace batch
+}  // namespace gfft

@@ -60,7 +60,8 @@ typedef enum {
   GFFT_ERR_DEVICE_MISMATCH = 8,   /* executed with another current device than the plan's */
   GFFT_ERR_BUFFER_TOO_SMALL = 9,  /* text query buffer too small */
   GFFT_ERR_OVER_BUDGET = 10,      /* serving: one request needs more HBM than the whole budget */
-  GFFT_ERR_TIMEOUT = 11           /* serving: gfft_server_wait ran out of time */
+  GFFT_ERR_TIMEOUT = 11,          /* serving: gfft_server_wait ran out of time; batch: a persistent wait hit its bound */
+  GFFT_ERR_ABORTED = 12           /* batch: a persistent launch gave up its waits on gfft_batch_set_abort */
 } gfft_status;
 
 /* Version number (GFFT_VERSION of the built library). */
@@ -206,10 +207,29 @@ int gfft_query_plan(int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int
  * reference, which plans and runs one grid per process).  FP32; all edges of
  * a transform are one size of {32, 48, 64, 96, 128, 160, 192, 256} (Nx and
  * Ny may also be 1).  Each transform uses the layout of gfft_exec_r2c on its
- * own array.  Transforms of one size share every launch: one launch per
- * (size, axis pass) covers the tiles of all of them.
+ * own array.
+ *
+ * Two schedulers.  GFFT_BATCH_PERSISTENT (default): one persistent launch
+ * runs every axis pass of every transform, one workgroup per slot of a plan
+ * made at creation: small transforms are run whole by one workgroup each
+ * (several per workgroup, packed longest first), larger ones by a team of
+ * workgroups whose members own interleaved tile shares of each pass.  A
+ * team member waits only for the shares of its own transform's pass (per
+ * share state words, no grid-wide barrier) and takes over shares whose owner
+ * has not started, so completion never depends on how many workgroups are
+ * resident.  Every wait is bounded (gfft_batch_set_wait_limit) and gives up
+ * when the host sets the abort word (gfft_batch_set_abort); the launch then
+ * drains and reports through gfft_batch_status.  The last workgroup to leave
+ * resets the counters, so launches and graph replays need no clearing
+ * between them.  Executions of one batch must not overlap (they share the
+ * counters).
+ * GFFT_BATCH_LEVELS: one launch per (size, axis pass) covering the tiles of
+ * every transform of that size, the sizes' chains on internal streams.
  * ------------------------------------------------------------------------- */
 typedef struct gfft_batch_s* gfft_batch;
+
+#define GFFT_BATCH_PERSISTENT 0
+#define GFFT_BATCH_LEVELS 1
 
 /* GFFT_OK if (nx, ny, nz, prec) is supported by the batched kernels. */
 int gfft_batch_query(int64_t nx, int64_t ny, int64_t nz, int prec);
@@ -226,27 +246,32 @@ int gfft_batch_set_data(gfft_batch batch, void* const* ptrs);
  * Synchronises the device first. */
 int gfft_batch_set_scales(gfft_batch batch, const double* r2c_scale, const double* c2r_scale);
 
-/* mode 0 = R2C of every transform, 1 = C2R, 2 = R2C then C2R.  The launches
- * of the different sizes run on internal streams forked from and joined
- * back into `stream`: work queued on `stream` later starts after the batch. */
+/* mode 0 = R2C of every transform, 1 = C2R, 2 = R2C then C2R, on `stream`
+ * (persistent: one launch; levels: the sizes' launches run on internal
+ * streams forked from and joined back into `stream`).  Work queued on
+ * `stream` later starts after the batch. */
 int gfft_batch_exec(gfft_batch batch, int mode, gfft_stream_t stream);
 
 /* Same work as one hipGraph launch on `stream` (built on first use per
- * mode, one branch per size).  The graph reads pointers and scales from
- * device memory, so later gfft_batch_set_data / _set_scales calls take
- * effect in replays. */
+ * mode: the persistent kernel as its one node, or one branch of level
+ * launches per size).  The graph reads pointers and scales from device
+ * memory, so later gfft_batch_set_data / _set_scales calls take effect in
+ * replays. */
 int gfft_batch_exec_graph(gfft_batch batch, int mode, gfft_stream_t stream);
 
-/* Autotuner of the batched kernels: every size class present has compiled
- * variants (radix order; points per thread, which sets the pencils per
- * 256-thread tile).  Times each variant's level chain of `mode`, `reps`
- * times, on the arrays set by gfft_batch_set_data (contents destroyed) and
- * keeps the fastest per class.  Synchronises `stream`; later graph launches
- * are rebuilt with the chosen kernels. */
+/* Autotuner of the batched kernels.  Tunes both schedulers, then keeps the
+ * one whose tuned execution of `mode` is faster (gfft_batch_scheduler tells
+ * which; gfft_batch_tuned_ms gives both times).  Persistent: every variant
+ * set (one compiled variant per size: radix order and points per thread,
+ * which sets the pencils per 256-thread tile) and team factor in {1, 2, 4},
+ * each timed as `reps` eager launches.  Levels: every size's variants and
+ * the per-transform-plan choice, each size's level chain timed alone.  Runs
+ * on the arrays set by gfft_batch_set_data (contents destroyed).
+ * Synchronises `stream`; later graph launches are rebuilt. */
 int gfft_batch_tune(gfft_batch batch, int mode, int reps, gfft_stream_t stream);
 
-/* Select variant `variant` (0 = default) of the size class with edge `edge`;
- * query the current choice.  set synchronises the device. */
+/* Level scheduler: select variant `variant` (0 = default) of the size class
+ * with edge `edge`; query the current choice.  set synchronises the device. */
 int gfft_batch_set_variant(gfft_batch batch, int64_t edge, int variant);
 int gfft_batch_variant(gfft_batch batch, int64_t edge, int* variant);
 
@@ -255,6 +280,42 @@ int gfft_batch_data_bytes(gfft_batch batch, int index, size_t* bytes);
 /* Tiles one execution of `mode` processes, and (if launches is not NULL)
  * the number of kernel launches it takes. */
 int gfft_batch_tiles(gfft_batch batch, int mode, uint32_t* tiles, uint32_t* launches);
+
+/* Scheduler: GFFT_BATCH_PERSISTENT (default) or GFFT_BATCH_LEVELS. */
+int gfft_batch_set_scheduler(gfft_batch batch, int scheduler);
+int gfft_batch_scheduler(gfft_batch batch, int* scheduler);
+
+/* Persistent scheduler: variant set (0..1) and team factor (1..64: a
+ * transform gets a team when its work exceeds that many slot shares, a slot
+ * share being the batch's work divided by the slots); query them and (if
+ * variants is not NULL) the set's variant per size class, 8 ints for the
+ * sizes 32 ... 256.  set rebuilds the plans and synchronises the device. */
+int gfft_batch_set_persist(gfft_batch batch, int set, int team_factor);
+int gfft_batch_persist(gfft_batch batch, int* set, int* team_factor, int* variants);
+
+/* The persistent plan of `mode`: slots (workgroups of the launch), teams
+ * (transforms run by two or more workgroups) and their members. */
+int gfft_batch_plan_info(gfft_batch batch, int mode, uint32_t* slots, uint32_t* teams, uint32_t* members);
+
+/* ms per eager execution of the tuned mode measured by the last
+ * gfft_batch_tune for each scheduler (-1 before any tune). */
+int gfft_batch_tuned_ms(gfft_batch batch, float* persistent_ms, float* levels_ms);
+
+/* Bound of one wait of the persistent kernel, seconds of device wall clock
+ * (default 2).  A wait that reaches it marks the launch failed. */
+int gfft_batch_set_wait_limit(gfft_batch batch, double seconds);
+
+/* Nonzero: every wait of the batch's persistent launches, running or later,
+ * gives up at once (the launch drains without computing the waiting work and
+ * reports GFFT_ERR_ABORTED); 0 clears the request.  Host memory only, no
+ * synchronisation, callable while a launch runs. */
+int gfft_batch_set_abort(gfft_batch batch, int abort);
+
+/* Synchronises the device, then reports and clears the sticky outcome of
+ * the persistent launches since the last call: GFFT_OK, GFFT_ERR_TIMEOUT (a
+ * wait reached its bound) or GFFT_ERR_ABORTED.  Their results are undefined
+ * unless it is GFFT_OK. */
+int gfft_batch_status(gfft_batch batch, int* status);
 
 int gfft_batch_destroy(gfft_batch batch);
 

@@ -1,8 +1,12 @@
-"""Batched mode: many transforms of uneven sizes executed as one unit.
-Each result is checked against the FP64 oracle (FP32 tolerance 1e-6), is
-bitwise the same alone and in the mix and the same as the single-transform
-plan's, and graph replays (with eager work between them) reproduce the eager
-result."""
+"""Batched mode: many transforms of uneven sizes executed as one unit, by
+the persistent scheduler (default: one launch over a work queue with
+per-pass completion counters) or the level scheduler (one launch per size
+and axis pass).  Each result is checked against the FP64 oracle (FP32
+tolerance 1e-6), is bitwise the same alone and in the mix and the same as
+the single-transform plan's, and graph replays (with eager work between
+them) reproduce the eager result.  The persistent kernel's bounded waits are
+exercised from the host: an abort request and a zero wait bound make every
+launch drain and report, and the next launch is correct again."""
 import sys
 
 import numpy as np
@@ -143,6 +147,9 @@ def test_eager_on_side_stream_orders_with_caller():
 def test_tile_and_launch_counts():
     dims = [(64, 64, 64), (1, 64, 64), (32, 32, 32)]
     with G.Batch(dims) as b:
+        assert b.scheduler() == G.Batch.PERSISTENT
+        assert b.tiles(G.ROUND_TRIP)[1] == 1  # one persistent launch
+        b.set_scheduler(G.Batch.LEVELS)
         tiles, launches = b.tiles(G.R2C)
         # class 32: P=8 -> 4 threads/row -> 64 pairs or columns per tile
         # class 64: P=8 -> 8 threads/row -> 32 pairs or columns per tile
@@ -180,6 +187,7 @@ def test_every_variant_matches_oracle(variant):
     bufs, host = inputs(torch, dims, 300 + variant)
     say(f"case batch variant {variant}, {len(dims)} transforms")
     with G.Batch(dims) as b:
+        b.set_scheduler(G.Batch.LEVELS)
         for n in SIZES:
             b.set_variant(n, variant)
         assert set(b.variants().values()) == {variant}
@@ -199,7 +207,13 @@ def test_tuned_batch_matches_oracle():
     with G.Batch(dims) as b:
         b.set_data(scratch)
         chosen = b.tune(G.ROUND_TRIP, reps=2)
-        assert set(chosen) == {d[2] for d in dims} and all(0 <= v <= 3 for v in chosen.values())
+        assert chosen["scheduler"] in ("persistent", "levels")
+        assert chosen["tuned_ms"]["persistent"] > 0 and chosen["tuned_ms"]["levels"] > 0
+        faster = "persistent" if chosen["tuned_ms"]["persistent"] <= chosen["tuned_ms"]["levels"] else "levels"
+        assert chosen["scheduler"] == faster
+        p = chosen["persist"]
+        assert 0 <= p["set"] <= 1 and p["team_factor"] in (1, 2, 4)
+        assert set(chosen["variants"]) == {d[2] for d in dims}
         bufs, host = inputs(torch, dims, 600)
         b.set_data(bufs)
         b.exec_graph(G.R2C)
@@ -217,6 +231,7 @@ def test_plain_variant_in_graph_follows_new_data_and_scales():
     import torch
     dims = mix(10, seed=8, max_edge=128) + [(160, 160, 160)]
     with G.Batch(dims) as b:
+        b.set_scheduler(G.Batch.LEVELS)
         for n in SIZES:
             b.set_variant(n, G.Batch.PLAIN)
         for rep, sc in enumerate((1.0, 0.5)):
@@ -280,3 +295,124 @@ def test_bench_mix_256_seed0_matches_oracle():
             err = O.rel_l2(buf.cpu().numpy()[:, :, :nz], fp[:, :, :nz])
             assert err <= 1e-6, (i, (nx, ny, nz), err)
     say(f"  worst R2C rel L2 {worst:.2e}")
+
+
+@pytest.mark.parametrize("set_index,chunk", [(0, 1), (0, 3), (1, 2), (1, 8)])
+def test_persistent_sets_and_chunks_match_oracle(set_index, chunk):
+    """Every variant set of the persistent kernel, with several team factors
+    (teams of many members down to all-solo plans), R2C against the FP64
+    oracle and the normalised round trip against the input (all sizes,
+    length-1 axes)."""
+    import torch
+    dims = mix(30, seed=9) + [(256, 256, 256), (192, 192, 192), (1, 1, 48)]
+    bufs, host = inputs(torch, dims, 800 + set_index)
+    say(f"case persistent set {set_index} chunk {chunk}, {len(dims)} transforms")
+    with G.Batch(dims) as b:
+        b.set_persist(set_index, chunk)
+        assert b.persist()["set"] == set_index and b.persist()["team_factor"] == chunk
+        info = b.plan_info(G.R2C)
+        say(f"  plan {info}")
+        assert info["slots"] >= 1 and (info["teams"] == 0) == (info["members"] == 0)
+        b.set_data(bufs)
+        b.exec(G.R2C)
+        torch.cuda.synchronize()
+        assert b.status() == 0
+        for (nx, ny, nz), buf, fp in zip(dims, bufs, host):
+            got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
+            err = O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), nz))
+            assert err <= 1e-6, ((nx, ny, nz), err)
+        b.set_scales(c2r=[1.0 / (nx * ny * nz) for nx, ny, nz in dims])
+        b.exec_graph(G.C2R)
+        torch.cuda.synchronize()
+        assert b.status() == 0
+        for (nx, ny, nz), buf, fp in zip(dims, bufs, host):
+            err = O.rel_l2(buf.cpu().numpy()[:, :, :nz], fp[:, :, :nz])
+            assert err <= 1e-6, ((nx, ny, nz), err)
+
+
+def test_persistent_matches_level_scheduler():
+    """Set 1 runs variant 1 of every size; the level scheduler with variant 1
+    forced for every size runs the same butterflies and radix order (the
+    persistent kernel pipelines the tiles; multiply-add contraction may
+    differ between the two instantiations): R2C and round trip agree to FP32
+    rounding (rel L2 <= 1e-6 on the transform region), and the R2C matches
+    the FP64 oracle."""
+    import torch
+    dims = mix(40, seed=10) + [(128, 1, 128)]
+    bufs, host = inputs(torch, dims, 900)
+    for mode in (G.R2C, G.ROUND_TRIP):
+        a = [x.clone() for x in bufs]
+        c = [x.clone() for x in bufs]
+        with G.Batch(dims) as b:
+            b.set_persist(1, 4)
+            b.set_data(a)
+            b.exec_graph(mode)
+            torch.cuda.synchronize()
+            assert b.status() == 0
+            b.set_scheduler(G.Batch.LEVELS)
+            for n in SIZES:
+                b.set_variant(n, 1)
+            b.set_data(c)
+            b.exec(mode)
+            torch.cuda.synchronize()
+        for x, y, d, fp in zip(a, c, dims, host):
+            nx, ny, nz = d
+            if mode == G.R2C:
+                gx = G.complex_view(x, nx, ny, nz).cpu().numpy()
+                gy = G.complex_view(y, nx, ny, nz).cpu().numpy()
+                assert O.rel_l2(gx, gy) <= 1e-6, (mode, d)
+                assert O.rel_l2(gx, O.r2c_p1(fp.astype(np.float64), nz)) <= 1e-6, (mode, d)
+            else:
+                assert O.rel_l2(x.cpu().numpy()[..., :nz], y.cpu().numpy()[..., :nz]) <= 1e-6, (mode, d)
+
+
+def test_persistent_abort_drains_and_reports():
+    """With the host's abort word set, every team wait gives up at once:
+    the launch still ends, reports GFFT_ERR_ABORTED once, and after the word
+    is cleared the next launch is right."""
+    import torch
+    dims = mix(20, seed=11, max_edge=128)
+    bufs, host = inputs(torch, dims, 950)
+    with G.Batch(dims) as b:
+        b.set_persist(0, 1)  # team factor 1: the larger transforms get teams, whose levels wait
+        assert b.plan_info(G.ROUND_TRIP)["teams"] > 0
+        b.set_data(bufs)
+        b.set_abort(True)
+        b.exec(G.ROUND_TRIP)
+        torch.cuda.synchronize()
+        assert b.status() == G.ERR_ABORTED
+        assert b.status() == 0  # cleared by the first read
+        b.set_abort(False)
+        work = [torch.from_numpy(h).to("cuda:0") for h in host]
+        b.set_data(work)
+        b.exec(G.R2C)
+        torch.cuda.synchronize()
+        assert b.status() == 0
+        for (nx, ny, nz), buf, fp in zip(dims, work, host):
+            got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
+            assert O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), nz)) <= 1e-6
+
+
+def test_persistent_zero_wait_bound_drains():
+    """A zero wait bound: a wait that is not satisfied at once gives up, the
+    launch drains and reports a timeout (or OK if no wait had to wait); with
+    the default bound back, results are right."""
+    import torch
+    dims = mix(30, seed=12) + [(256, 256, 256)]
+    bufs, host = inputs(torch, dims, 970)
+    with G.Batch(dims) as b:
+        b.set_data(bufs)
+        b.set_wait_limit(0.0)
+        for _ in range(3):
+            b.exec_graph(G.ROUND_TRIP)
+        torch.cuda.synchronize()
+        assert b.status() in (0, G.ERR_TIMEOUT)
+        b.set_wait_limit(2.0)
+        work = [torch.from_numpy(h).to("cuda:0") for h in host]
+        b.set_data(work)
+        b.exec_graph(G.R2C)
+        torch.cuda.synchronize()
+        assert b.status() == 0
+        for (nx, ny, nz), buf, fp in zip(dims, work, host):
+            got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
+            assert O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), nz)) <= 1e-6
