@@ -20,13 +20,14 @@ import os
 __all__ = [
     "GfftError", "Plan", "lib", "lib_path", "query_plan", "kdim", "alloc_inplace",
     "real_view", "complex_view", "F32", "F64", "tune_cache_text", "load_tune_cache_text", "clear_tune_cache",
-    "save_tune_cache", "load_tune_cache", "ERR_TIMEOUT", "ERR_ABORTED",
+    "save_tune_cache", "load_tune_cache", "ERR_TIMEOUT", "ERR_ABORTED", "ERR_SCRATCH_TOO_SMALL",
 ]
 
 F32 = 0
 F64 = 1
 ERR_TIMEOUT = 11  # GFFT_ERR_TIMEOUT
 ERR_ABORTED = 12  # GFFT_ERR_ABORTED
+ERR_SCRATCH_TOO_SMALL = 13  # GFFT_ERR_SCRATCH_TOO_SMALL
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -326,7 +327,7 @@ def _bind_batch(L):
     L.gfft_batch_set_data.argtypes = [vp, ctypes.POINTER(vp)]
     L.gfft_batch_set_scales.restype = ctypes.c_int
     L.gfft_batch_set_scales.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
-    for name in ("gfft_batch_exec", "gfft_batch_exec_graph"):
+    for name in ("gfft_batch_exec", "gfft_batch_exec_graph", "gfft_batch_exec_noalloc"):
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [vp, ctypes.c_int, vp]
@@ -401,6 +402,13 @@ class Batch:
 
     def exec_graph(self, mode=ROUND_TRIP, stream=None) -> None:
         _check(lib().gfft_batch_exec_graph(self.handle, int(mode), _stream_handle(stream)), "gfft_batch_exec_graph")
+
+    def exec_noalloc(self, mode=ROUND_TRIP, stream=None) -> None:
+        """The persistent scheduler's run of ``mode`` with its counters in the
+        library's static per-GPU array: allocates nothing.  Calls that overlap
+        on one GPU race on that array."""
+        _check(lib().gfft_batch_exec_noalloc(self.handle, int(mode), _stream_handle(stream)),
+               "gfft_batch_exec_noalloc")
 
     def tune(self, mode=ROUND_TRIP, reps: int = 3, stream=None) -> dict:
         """Autotune both schedulers on the current data (destroyed) and keep

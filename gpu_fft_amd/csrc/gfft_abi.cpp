@@ -54,6 +54,7 @@ const char* gfft_strerror(int status) {
     case GFFT_ERR_OVER_BUDGET: return "request larger than the server's whole HBM budget";
     case GFFT_ERR_TIMEOUT: return "timed out (server drain, or a bounded wait of a persistent batch launch)";
     case GFFT_ERR_ABORTED: return "a persistent batch launch gave up its waits on the host's abort request";
+    case GFFT_ERR_SCRATCH_TOO_SMALL: return "the plan needs more counter words than the library's static scratch array holds";
     default: return "unknown status";
   }
 }

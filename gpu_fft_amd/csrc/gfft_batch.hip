@@ -387,6 +387,7 @@ gfft::batch::PersistArgs persist_args(const gfft_batch_s* b, int mode) {
   a.nslots = dp.nslots;
   a.claim_words = dp.claim_words;
   a.ctr = b->d_ctr;
+  a.status = b->d_ctr + gfft::kCtrStatus;
   a.host_abort = b->h_abort;
   const double ticks = b->wait_limit_s * (double)b->wall_khz * 1e3;
   a.wait_ticks = ticks >= 1.8e19 ? ~0ull : (uint64_t)ticks;
@@ -771,6 +772,22 @@ int gfft_batch_exec(gfft_batch b, int mode, gfft_stream_t stream) {
   if (st != GFFT_OK) return st;
   if (b->sched == GFFT_BATCH_PERSISTENT) return launch_persistent(b, mode, reinterpret_cast<hipStream_t>(stream));
   return launch_eager(b, mode, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gfft_batch_exec_noalloc(gfft_batch b, int mode, gfft_stream_t stream) {
+  if (!b || mode < 0 || mode > 2 || !b->data_set) return GFFT_ERR_INVALID_ARG;
+  int st = check_dev(b);
+  if (st != GFFT_OK) return st;
+  const gfft_batch_s::DevPlan& dp = b->plan[mode];
+  if (dp.nslots == 0) return GFFT_OK;
+  if (gfft::kCtrShares + (size_t)dp.claim_words > gfft::batch::kStaticCounterWords) return GFFT_ERR_SCRATCH_TOO_SMALL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  uint32_t* ctr = nullptr;
+  if (gfft::batch::static_counters(&ctr, s) != hipSuccess) return GFFT_ERR_HIP;
+  gfft::batch::PersistArgs a = persist_args(b, mode);
+  a.ctr = ctr;  // the launch's counters; failures still reach the batch's status word
+  return gfft::batch::launch_persist(b->pset, persist_grid(b, mode), b->plds, a, s) == hipSuccess ? GFFT_OK
+                                                                                                   : GFFT_ERR_HIP;
 }
 
 int gfft_batch_exec_graph(gfft_batch b, int mode, gfft_stream_t stream) {

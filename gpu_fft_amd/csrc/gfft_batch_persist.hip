@@ -26,11 +26,24 @@
 // to leave moves the failure bits into the sticky status word and zeroes the
 // other counter words, so the next launch (or graph replay) starts from zero
 // without a memset.
+#include <map>
+#include <mutex>
+
 #include "gfft_batch_tiles.h"
 
 namespace gfft {
 namespace batch {
+
+// Counter words of gfft_batch_exec_noalloc (see static_counters): every
+// call on one GPU shares this array, which keeps the last call's contents
+// (all zero after a launch completes: its last workgroup resets them).
+static __device__ __align__(256) uint32_t scratch[kStaticCounterWords];
+
 namespace {
+
+__global__ void k_fill_zero(uint32_t* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
 
 template <int S, int C>
 using SCfg = BCfg<C, kSetVariant[S][C]>;
@@ -311,12 +324,12 @@ __global__ __launch_bounds__(kBT) void k_batch_persist(PersistArgs a) {
   if (s_last) {
     if (tid == 0) {
       const uint32_t f = ld_agent(a.ctr + kCtrFail);
-      if (f) __hip_atomic_fetch_or(a.ctr + kCtrStatus, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (f) __hip_atomic_fetch_or(a.status, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     const uint32_t words = kCtrShares + a.claim_words;
     for (uint32_t i = tid; i < words; i += kBT)
-      if (i != kCtrStatus) __hip_atomic_store(a.ctr + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a.ctr + i != a.status) __hip_atomic_store(a.ctr + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -341,6 +354,30 @@ const void* persist_kernel(int set) {
 
 size_t persist_lds_bytes(int set, const bool* present) {
   return set == 0 ? lds_of_set<0>(present) : lds_of_set<1>(present);
+}
+
+hipError_t static_counters(uint32_t** out, hipStream_t s) {
+  static std::mutex mu;
+  static std::map<int, uint32_t*> ptr;  // per device: the array's device address
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = ptr.find(dev);
+  if (it == ptr.end()) {
+    uint32_t* p = nullptr;
+    e = hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    // first use on this device: clear it with a kernel (stream-ordered, and a
+    // kernel node if the caller captures the stream)
+    hipLaunchKernelGGL(k_fill_zero, dim3(64), dim3(256), 0, s, p, kStaticCounterWords);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    it = ptr.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return hipSuccess;
 }
 
 hipError_t launch_persist(int set, uint32_t grid, size_t lds, const PersistArgs& a, hipStream_t s) {

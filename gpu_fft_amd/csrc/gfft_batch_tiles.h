@@ -170,6 +170,7 @@ struct PersistArgs {
   uint32_t nslots;
   uint32_t claim_words;
   uint32_t* ctr;               // counter words (kCtrShares + claim_words)
+  uint32_t* status;            // sticky status word the failure bits are ORed into
   const uint32_t* host_abort;  // pinned host word: nonzero = every wait gives up
   uint64_t wait_ticks;         // bound of one wait, device wall-clock ticks
   uint64_t steal_ticks;        // a waiting member steals unclaimed shares after this long
@@ -181,6 +182,14 @@ const void* persist_kernel(int set);
 // dynamic LDS bytes of set `set` for the classes marked present
 size_t persist_lds_bytes(int set, const bool* present);
 hipError_t launch_persist(int set, uint32_t grid, size_t lds, const PersistArgs& a, hipStream_t s);
+
+// Counter words of the allocation-free entry point: a file-scope device
+// array in gfft_batch_persist.hip, one per GPU, held until the process ends.
+constexpr size_t kStaticCounterWords = size_t(1) << 16;  // 256 KiB
+// Device address of the calling device's array (hipGetSymbolAddress once per
+// device, cached); zeroes it with a fill kernel on `s` at the first call on a
+// device.  Returns a HIP status.
+hipError_t static_counters(uint32_t** out, hipStream_t s);
 
 }  // namespace batch
 }  // namespace gfft

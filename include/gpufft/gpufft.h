@@ -61,7 +61,8 @@ typedef enum {
   GFFT_ERR_BUFFER_TOO_SMALL = 9,  /* text query buffer too small */
   GFFT_ERR_OVER_BUDGET = 10,      /* serving: one request needs more HBM than the whole budget */
   GFFT_ERR_TIMEOUT = 11,          /* serving: gfft_server_wait ran out of time; batch: a persistent wait hit its bound */
-  GFFT_ERR_ABORTED = 12           /* batch: a persistent launch gave up its waits on gfft_batch_set_abort */
+  GFFT_ERR_ABORTED = 12,          /* batch: a persistent launch gave up its waits on gfft_batch_set_abort */
+  GFFT_ERR_SCRATCH_TOO_SMALL = 13 /* batch: gfft_batch_exec_noalloc's plan exceeds the static counter array */
 } gfft_status;
 
 /* Version number (GFFT_VERSION of the built library). */
@@ -251,6 +252,19 @@ int gfft_batch_set_scales(gfft_batch batch, const double* r2c_scale, const doubl
  * streams forked from and joined back into `stream`).  Work queued on
  * `stream` later starts after the batch. */
 int gfft_batch_exec(gfft_batch batch, int mode, gfft_stream_t stream);
+
+/* No counterpart in the reference; for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid it).  The persistent scheduler's execution of `mode` (whatever
+ * the batch's current scheduler), with its counter words in a static array
+ * of 65536 words that the library holds on each GPU until the process ends
+ * (device address taken once per device with hipGetSymbolAddress; zeroed by a
+ * fill kernel on the first call on a device).  Calls no allocating routine.
+ * A plan needing more than 65532 share words returns
+ * GFFT_ERR_SCRATCH_TOO_SMALL.  Every call on one GPU shares the array:
+ * calls that overlap on one GPU (other streams, other batches) race.
+ * Failures are reported by gfft_batch_status like those of gfft_batch_exec. */
+int gfft_batch_exec_noalloc(gfft_batch batch, int mode, gfft_stream_t stream);
 
 /* Same work as one hipGraph launch on `stream` (built on first use per
  * mode: the persistent kernel as its one node, or one branch of level

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_strerror_covers_codes():
-    for code in range(0, 13):
+    for code in range(0, 14):
         s = G.lib().gfft_strerror(code).decode()
         assert s and s != "unknown status"
     assert G.lib().gfft_strerror(99).decode() == "unknown status"
