@@ -143,9 +143,11 @@ struct gfft_batch_s {
   struct DevPlan {
     gfft::PSeg* seg = nullptr;
     gfft::PTeam* team = nullptr;
-    gfft::PSlot* slot = nullptr;
-    gfft::PEntry* entry = nullptr;
-    uint32_t nslots = 0, claim_words = 0, tiles = 0, teams_k2 = 0, members = 0;
+    gfft::PEntry* local = nullptr;
+    uint32_t* solo = nullptr;
+    uint32_t* team_ids = nullptr;
+    uint32_t nslots = 0;  // workgroups of the launch (domains * per_dom; 0: nothing to run)
+    uint32_t domains = 1, per_dom = 1, nsolo = 0, claim_words = 0, tiles = 0, teams_k2 = 0, members = 0;
   } plan[3];
   uint32_t* d_ctr = nullptr;          // counter words, sized for the largest plan
   size_t ctr_words = 0;
@@ -154,6 +156,7 @@ struct gfft_batch_s {
   int wall_khz = 0;                   // device wall-clock rate
   size_t plds = 0;                    // dynamic LDS of the current set for the classes present
   int pocc = 1;                       // resident persistent workgroups per CU
+  int xccs = 1;                       // L2 domains (XCDs) of the device
 };
 
 namespace {
@@ -161,6 +164,21 @@ namespace {
 const int kModeAxis[3][gfft::batch::kMaxLevels] = {{0, 1, 2}, {2, 1, 0}, {0, 1, 2, 2, 1, 0}};
 const int kModeInv[3][gfft::batch::kMaxLevels] = {{0, 0, 0}, {1, 1, 1}, {0, 0, 0, 1, 1, 1}};
 const int kModeLevels[3] = {3, 3, 6};
+// Shares per team member and level in the persistent plans: a member runs
+// its own in turn, so a slow member's later ones can be taken by the others.
+constexpr uint32_t kSharesPerMember = 2;
+
+// L2 domains (XCDs) of the current device; 1 where the count is unknown or
+// above what a plan holds (the kernel then gives team roles only to the
+// workgroups of XCC 0, which share one L2).
+int device_xcc_count() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 1;
+  }
+  return n >= 1 && n <= (int)gfft::kMaxDomains ? n : 1;
+}
 
 uint32_t tiles_of(const BXform& x, int axis, int w) {
   if (x.cls[axis] < 0) return 0;
@@ -195,8 +213,9 @@ void free_plans(gfft_batch_s* b) {
   for (auto& p : b->plan) {
     if (p.seg) (void)hipFree(p.seg);
     if (p.team) (void)hipFree(p.team);
-    if (p.slot) (void)hipFree(p.slot);
-    if (p.entry) (void)hipFree(p.entry);
+    if (p.local) (void)hipFree(p.local);
+    if (p.solo) (void)hipFree(p.solo);
+    if (p.team_ids) (void)hipFree(p.team_ids);
     p = gfft_batch_s::DevPlan{};
   }
 }
@@ -321,7 +340,8 @@ int build_plans(gfft_batch_s* b) {
       occ < 1)
     occ = 1;
   b->pocc = occ;
-  const uint32_t slots = (uint32_t)std::max(1, b->cus * occ);
+  const uint32_t domains = (uint32_t)b->xccs;
+  const uint32_t per_dom = std::max(1u, (uint32_t)std::max(1, b->cus * occ) / domains);
   size_t need = 0;
   for (int m = 0; m < 3; ++m) {
     PLevelSpec lv[kMaxLevels];
@@ -339,18 +359,22 @@ int build_plans(gfft_batch_s* b) {
           for (int l = 0; l < kModeLevels[m]; ++l) levels += tiles(t, l) ? 1.0 : 0.0;
           return (double)x.nrows * x.K * levels;
         },
-        slots, (double)b->pteam);
+        domains, per_dom, (double)b->pteam, 2, kSharesPerMember);
     if (check_plan(pl) != 0) return GFFT_ERR_INVALID_ARG;  // cannot happen: build_plan's construction
     gfft_batch_s::DevPlan& dp = b->plan[m];
     int st = upload_vec(&dp.seg, pl.segs);
     if (st == GFFT_OK) st = upload_vec(&dp.team, pl.teams);
-    if (st == GFFT_OK) st = upload_vec(&dp.slot, pl.slots);
-    if (st == GFFT_OK) st = upload_vec(&dp.entry, pl.entries);
+    if (st == GFFT_OK) st = upload_vec(&dp.local, pl.local);
+    if (st == GFFT_OK) st = upload_vec(&dp.solo, pl.solo);
+    if (st == GFFT_OK) st = upload_vec(&dp.team_ids, pl.team_ids);
     if (st != GFFT_OK) {
       free_plans(b);
       return st;
     }
-    dp.nslots = (uint32_t)pl.slots.size();
+    dp.nslots = pl.segs.empty() ? 0u : pl.domains * pl.per_dom;
+    dp.domains = pl.domains;
+    dp.per_dom = pl.per_dom;
+    dp.nsolo = (uint32_t)pl.solo.size();
     dp.claim_words = pl.claim_words;
     for (const PSeg& sg : pl.segs) dp.tiles += sg.ntiles;
     for (const PTeam& tm : pl.teams)
@@ -382,16 +406,20 @@ gfft::batch::PersistArgs persist_args(const gfft_batch_s* b, int mode) {
   a.xf = b->d_xf;
   a.seg = dp.seg;
   a.team = dp.team;
-  a.slot = dp.slot;
-  a.entry = dp.entry;
-  a.nslots = dp.nslots;
+  a.local = dp.local;
+  a.solo = dp.solo;
+  a.team_ids = dp.team_ids;
+  a.domains = dp.domains;
+  a.per_dom = dp.per_dom;
+  a.nsolo = dp.nsolo;
+  a.nteams = dp.teams_k2;
   a.claim_words = dp.claim_words;
   a.ctr = b->d_ctr;
   a.status = b->d_ctr + gfft::kCtrStatus;
   a.host_abort = b->h_abort;
   const double ticks = b->wait_limit_s * (double)b->wall_khz * 1e3;
   a.wait_ticks = ticks >= 1.8e19 ? ~0ull : (uint64_t)ticks;
-  a.steal_ticks = (uint64_t)(20e-6 * (double)b->wall_khz * 1e3);  // 20 us
+  a.steal_ticks = 0;  // a member steals as soon as its own shares are done
   for (int c = 0; c < kClasses; ++c)
     a.tw[c] = static_cast<const gfft::cx<float>*>(b->d_tw[c][kSetVariant[b->pset][c]]);
   return a;
@@ -666,6 +694,7 @@ int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec)
     }
   }
   b->cus = device_cu_count();
+  b->xccs = device_xcc_count();
   int st = b->cus > 0 ? GFFT_OK : GFFT_ERR_HIP;
   auto alloc = [&](void** p, size_t n) {
     if (st == GFFT_OK && hipMalloc(p, n) != hipSuccess) {

@@ -1,31 +1,37 @@
 // Persistent batched kernel: every axis pass of every transform of a batch
 // mode in one launch, eager or captured in a graph, following the plan of
-// gfft_batch_sched.h (one workgroup per slot; solo transforms run whole by
-// one workgroup, larger ones by a team whose members own interleaved tile
-// shares of each level).
+// gfft_batch_sched.h.
 //
-// Solo transforms need no synchronisation beyond the workgroup barrier
-// between levels (one workgroup's stores are visible to its own later loads;
-// its data can stay in the L2 of its XCD across the six passes).
+// A workgroup reads its XCC id (its L2 domain) and takes the next local slot
+// of that domain (arrival counter).  If the slot holds a team member, it runs
+// the member's part of the team's levels; then it takes solo transforms from
+// the shared queue (one ticket each) and runs each whole, level after level.
+// A solo transform needs no synchronisation beyond the workgroup barrier
+// between levels (one workgroup's stores are visible to its own later loads).
 //
-// Teams synchronise per level through share state words (free, claimed,
-// done), also across the XCDs' separate L2 caches:
-//   run a share: claim it (compare-and-swap, agent scope), run its tiles,
-//             every thread an agent-scope release fence (L2 write-back of
-//             its stores), workgroup barrier, one agent-scope release store
-//             of "done" to the share's word;
-//   wait:     the first wavefront reads the level's share words (agent-scope
-//             atomic loads, 64 per instruction) until all are done, then an
-//             agent-scope acquire fence; barrier; every thread acquires too.
-// While waiting, after steal_ticks, a member claims and runs any share still
-// free (its owner has not started), so a team never depends on all of its
-// members being resident.  Every wait is bounded: it gives up when the
-// launch's wall-clock bound passes or when the host sets the batch's abort
-// word (pinned host memory, system-scope loads), and marks the launch failed;
-// a failed launch's workgroups stop computing and leave.  The last workgroup
-// to leave moves the failure bits into the sticky status word and zeroes the
-// other counter words, so the next launch (or graph replay) starts from zero
-// without a memset.
+// Every workgroup that touches a team's array runs in the team's domain, so
+// they all go through one L2, and a level's hand-over needs L2-level
+// ordering only (the ordering the memory model uses for a workgroup whose
+// waves are split over CUs of one XCD):
+//   run a share: claim it (compare-and-swap on the share's word), run its
+//             tiles, every thread waits for its stores to complete
+//             (s_waitcnt vmcnt(0): they are in the L2), workgroup barrier,
+//             one atomic store of "done" to the share's word;
+//   wait:     the first wavefront reads the level's share words (atomic
+//             loads, 64 per instruction) until all are done; barrier; every
+//             wavefront invalidates the CU's L1 (buffer_inv sc0) before it
+//             loads the next level.
+// No L2 write-back or invalidation happens inside the launch.  While
+// waiting, after steal_ticks, a member claims and runs any share still free
+// (its owner has not started), so a team never depends on all of its members
+// being resident.  Every wait is bounded: it gives up when the launch's
+// wall-clock bound passes or when the host sets the batch's abort word
+// (pinned host memory, system-scope loads), and marks the launch failed; a
+// failed launch's workgroups stop computing and leave.  The last workgroup to
+// leave runs the teams of which no member arrived (whole, alone), moves the
+// failure bits into the sticky status word and zeroes the other counter
+// words, so the next launch (or graph replay) starts from zero without a
+// memset.
 #include <map>
 #include <mutex>
 
@@ -148,8 +154,10 @@ struct Seq {
     }
   }
 
-  __device__ static void run(const BXform& x, int axis, int inv, uint32_t first, uint32_t step, uint32_t count,
-                             const cx<float>* tw, cx<float>* smem) {
+  // inlined into each set's class_tiles, so that each copy is compiled under
+  // its own kernel's register limit
+  __device__ __forceinline__ static void run(const BXform& x, int axis, int inv, uint32_t first, uint32_t step,
+                                             uint32_t count, const cx<float>* tw, cx<float>* smem) {
     cx<float>* stw = smem + F::lds_entries();
     for (int i = threadIdx.x; i < SC::twoff(SC::S); i += kBT) stw[i] = tw[i];
     __syncthreads();  // the table is complete (and the previous call's LDS reads are done)
@@ -200,15 +208,28 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
 
 __device__ __forceinline__ bool launch_failed(const PersistArgs& a) { return ld_agent(a.ctr + kCtrFail) != 0u; }
 
-// tiles m, m + k, ... of level sg, then publish the share as done
+// The XCC (L2 domain) this wavefront runs on.
+__device__ __forceinline__ uint32_t xcc_id() {
+  uint32_t v;
+  __asm__ volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return v & 0xfu;
+}
+
+// This wavefront's global stores have completed (they are in the L2).
+__device__ __forceinline__ void stores_complete() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Later loads of this CU miss the L1 (they see what the L2 holds).
+__device__ __forceinline__ void invalidate_l1() { __asm__ volatile("buffer_inv sc0" ::: "memory"); }
+
+// share m of level sg (tiles m, m + n, ... of n shares), then publish it as done
 template <int S>
-__device__ void run_share(const PersistArgs& a, const BXform& x, const PSeg& sg, uint32_t k, uint32_t m,
+__device__ void run_share(const PersistArgs& a, const BXform& x, const PSeg& sg, uint32_t n, uint32_t m,
                           cx<float>* smem) {
-  run_set_tiles<S>(x, sg, m, k, share_tiles(sg.ntiles, k, m), a, smem);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this thread's stores, written back
-  __syncthreads();                                    // every thread's
+  run_set_tiles<S>(x, sg, m, n, share_tiles(sg.ntiles, n, m), a, smem);
+  stores_complete();  // this thread's stores are in the L2
+  __syncthreads();    // every thread's
   if (threadIdx.x == 0)
-    __hip_atomic_store(a.ctr + kCtrShares + sg.claim0 + m, kShareDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(a.ctr + kCtrShares + sg.claim0 + m, kShareDone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ bool claim_share(const PersistArgs& a, const PSeg& sg, uint32_t m) {
@@ -220,11 +241,25 @@ __device__ __forceinline__ bool claim_share(const PersistArgs& a, const PSeg& sg
 // Wait-step outcomes, decided by the first wavefront
 constexpr uint32_t kStepDone = 0xffffffffu, kStepFail = 0xfffffffeu, kStepSpin = 0xfffffffdu;
 
-// First wavefront only (all 64 lanes): one look at the level's share words.
+// State of one workgroup's wait (first wavefront).
+struct WaitState {
+  uint64_t t0;       // wall clock at the start of the wait
+  uint64_t checked;  // wall clock of the last look at the failure and abort words (0: none yet)
+  uint32_t next;     // where the next steal attempt starts (the scan runs downwards)
+  uint32_t nap;      // sleep length of the next spin (grows to kMaxNap)
+};
+
+constexpr uint32_t kMaxNap = 16;            // x 512 clocks: about 4 us between looks at most
+constexpr uint64_t kCheckTicks = 1000;      // failure / abort words read at most every 10 us
+
+// First wavefront only (all 64 lanes): one look at the level's k share words.
 // Returns kStepDone, kStepFail, kStepSpin, or the index (< k) of a share
 // this workgroup has just claimed (steal).  Every decision is made by lane 0
-// and broadcast, so the wavefront's control flow stays uniform.
-__device__ uint32_t wait_step(const PersistArgs& a, const PSeg& sg, uint32_t k, uint64_t t0, uint32_t* next) {
+// and broadcast, so the wavefront's control flow stays uniform.  The
+// launch-wide words (failure bits, the host's abort word in host memory) are
+// read at most every kCheckTicks, and the sleep between looks grows, so the
+// waiting workgroups put little traffic on the memory system.
+__device__ __forceinline__ uint32_t wait_step(const PersistArgs& a, const PSeg& sg, uint32_t k, WaitState* ws) {
   const uint32_t lane = threadIdx.x;
   const uint32_t* w = a.ctr + kCtrShares + sg.claim0;
   bool mine_done = true, mine_free = false;
@@ -237,91 +272,140 @@ __device__ uint32_t wait_step(const PersistArgs& a, const PSeg& sg, uint32_t k, 
   const bool any_free = __ballot(mine_free) != 0ull;
   uint32_t out = kStepSpin;
   if (lane == 0) {
-    const uint64_t el = wall_clock64() - t0;
-    if (launch_failed(a)) {
-      out = kStepFail;
-    } else if (a.host_abort &&
-               __hip_atomic_load(a.host_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
-      __hip_atomic_fetch_or(a.ctr + kCtrFail, kFailAborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      out = kStepFail;
-    } else if (el > a.wait_ticks) {
-      __hip_atomic_fetch_or(a.ctr + kCtrFail, kFailTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      out = kStepFail;
-    } else if (any_free && el > a.steal_ticks) {
-      // steal the first free share at or after the rotating start *next*
+    const uint64_t now = wall_clock64();
+    const uint64_t el = now - ws->t0;
+    if (now - ws->checked > kCheckTicks || el > a.wait_ticks) {
+      ws->checked = now;
+      if (launch_failed(a)) {
+        out = kStepFail;
+      } else if (a.host_abort &&
+                 __hip_atomic_load(a.host_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+        __hip_atomic_fetch_or(a.ctr + kCtrFail, kFailAborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out = kStepFail;
+      } else if (el > a.wait_ticks) {
+        __hip_atomic_fetch_or(a.ctr + kCtrFail, kFailTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out = kStepFail;
+      }
+    }
+    if (out == kStepSpin && any_free && el > a.steal_ticks) {
+      // steal the first free share at or below the rotating start (owners
+      // run their shares upwards, so a slow member's last ones come first)
       for (uint32_t n = 0; n < k; ++n) {
-        uint32_t j = *next + n;
-        if (j >= k) j -= k;
+        const uint32_t j = ws->next >= n ? ws->next - n : ws->next + k - n;
         if (ld_agent(w + j) != kShareFree) continue;
         uint32_t expect = kShareFree;
         if (__hip_atomic_compare_exchange_strong(a.ctr + kCtrShares + sg.claim0 + j, &expect, kShareClaimed,
                                                  __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
           out = j;
-          *next = j + 1 < k ? j + 1 : 0u;
+          ws->next = j > 0 ? j - 1 : k - 1;
           break;
         }
       }
     }
   }
   out = __shfl(out, 0);
-  if (out == kStepSpin) __builtin_amdgcn_s_sleep(2);
+  if (out == kStepSpin) {
+    for (uint32_t i = 0; i < ws->nap; ++i) __builtin_amdgcn_s_sleep(8);
+    ws->nap = ws->nap < kMaxNap ? 2 * ws->nap : kMaxNap;
+  } else {
+    ws->nap = 1;
+  }
   return out;
 }
 
+// every level of transform t, by this workgroup alone
 template <int S>
-__global__ __launch_bounds__(kBT) void k_batch_persist(PersistArgs a) {
-  extern __shared__ cx<float> smem[];
-  __shared__ uint32_t s_step, s_last, s_claimed;
+__device__ void run_whole(const PersistArgs& a, uint32_t t, cx<float>* smem) {
+  const PTeam tm = a.team[t];
+  const BXform x = a.xf[t];
+  for (uint32_t s = tm.seg0; s < tm.seg0 + tm.nseg; ++s) {
+    const PSeg sg = a.seg[s];
+    run_set_tiles<S>(x, sg, 0, 1, sg.ntiles, a, smem);
+  }
+}
+
+// The kernel's body; sh: six LDS words (step, last, claimed, team, member, job).
+template <int S>
+__device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* smem, uint32_t* sh) {
+  uint32_t *s_step = sh, *s_last = sh + 1, *s_claimed = sh + 2, *s_team = sh + 3, *s_member = sh + 4,
+           *s_job = sh + 5;
   const uint32_t tid = threadIdx.x;
   bool failed = false;
-  if (blockIdx.x < a.nslots) {
-    const PSlot sl = a.slot[blockIdx.x];
-    for (uint32_t i = 0; i < sl.count && !failed; ++i) {
-      const PEntry e = a.entry[sl.first + i];
-      const PTeam tm = a.team[e.team];
-      const BXform x = a.xf[e.team];
-      for (uint32_t s = tm.seg0; s < tm.seg0 + tm.nseg && !failed; ++s) {
-        const PSeg sg = a.seg[s];
-        if (tm.k == 1) {  // solo: the whole level here, then on to the next
-          run_set_tiles<S>(x, sg, 0, 1, sg.ntiles, a, smem);
-          continue;
-        }
-        // team member: the own share, then wait (stealing) for the level
-        if (tid == 0) s_claimed = claim_share(a, sg, e.member) ? 1u : 0u;
-        __syncthreads();
-        if (s_claimed) run_share<S>(a, x, sg, tm.k, e.member, smem);
-        uint64_t t0 = 0;
-        uint32_t next = e.member + 1 < tm.k ? e.member + 1 : 0u;
-        if (tid < 64) t0 = wall_clock64();
-        for (;;) {
-          if (tid < 64) {
-            const uint32_t st = wait_step(a, sg, tm.k, t0, &next);
-            if (tid == 0) {
-              if (st == kStepDone) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-              s_step = st;
-            }
-          }
-          __syncthreads();
-          const uint32_t st = s_step;
-          __syncthreads();  // s_step is read before the next step writes it
-          if (st == kStepDone) break;
-          if (st == kStepFail) {
-            failed = true;
-            break;
-          }
-          if (st < tm.k) run_share<S>(a, x, sg, tm.k, st, smem);  // a stolen share
-        }
-        if (!failed) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every thread: the level's stores
+  // the next local slot of this workgroup's domain
+  if (tid == 0) {
+    const uint32_t d = xcc_id();
+    uint32_t team = kNoTeam, member = 0;
+    if (d < a.domains) {
+      const uint32_t li = __hip_atomic_fetch_add(a.ctr + kCtrArrive + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (li < a.per_dom) {
+        const PEntry e = a.local[d * a.per_dom + li];
+        team = e.team;
+        member = e.member;
       }
     }
-  }
-  // leave; the last workgroup out resets the counters for the next launch
-  if (tid == 0) {
-    const uint32_t prev = __hip_atomic_fetch_add(a.ctr + kCtrExited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = prev == gridDim.x - 1 ? 1u : 0u;
+    *s_team = team;
+    *s_member = member;
   }
   __syncthreads();
-  if (s_last) {
+  const uint32_t team = *s_team, member = *s_member;
+  if (team != kNoTeam) {  // a team member: per level the own share, then wait (stealing) for the level
+    const PTeam tm = a.team[team];
+    const BXform x = a.xf[team];
+    const uint32_t ns = tm.k * tm.q;  // shares of a level
+    for (uint32_t s = tm.seg0; s < tm.seg0 + tm.nseg && !failed; ++s) {
+      const PSeg sg = a.seg[s];
+      for (uint32_t i = 0; i < tm.q; ++i) {  // the own shares, in turn
+        const uint32_t own = member + i * tm.k;
+        if (tid == 0) *s_claimed = claim_share(a, sg, own) ? 1u : 0u;
+        __syncthreads();
+        const bool mine = *s_claimed != 0u;
+        __syncthreads();  // s_claimed is read before the next claim writes it
+        if (mine) run_share<S>(a, x, sg, ns, own, smem);
+      }
+      WaitState ws{0, 0, ns - 1 - member, 1};
+      if (tid < 64) ws.t0 = wall_clock64();  // checked = 0: the first look reads the launch-wide words
+      for (;;) {
+        if (tid < 64) {
+          const uint32_t st = wait_step(a, sg, ns, &ws);
+          if (tid == 0) *s_step = st;
+        }
+        __syncthreads();
+        const uint32_t st = *s_step;
+        __syncthreads();  // s_step is read before the next step writes it
+        if (st == kStepDone) break;
+        if (st == kStepFail) {
+          failed = true;
+          break;
+        }
+        if (st < ns) run_share<S>(a, x, sg, ns, st, smem);  // a stolen share
+      }
+      if (!failed) invalidate_l1();  // the level's stores, from the L2
+    }
+  }
+  // solo transforms from the shared queue
+  while (!failed) {
+    if (tid == 0)
+      *s_job = launch_failed(a) ? kNoTeam
+                                : __hip_atomic_fetch_add(a.ctr + kCtrSolo, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t j = *s_job;
+    __syncthreads();  // s_job is read before the next ticket writes it
+    if (j >= a.nsolo) break;
+    run_whole<S>(a, a.solo[j], smem);
+  }
+  // leave; the last workgroup out runs the orphaned teams and resets the counters
+  if (tid == 0) {
+    const uint32_t prev = __hip_atomic_fetch_add(a.ctr + kCtrExited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    *s_last = prev == gridDim.x - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (*s_last) {
+    if (!launch_failed(a))
+      for (uint32_t i = 0; i < a.nteams; ++i) {
+        const uint32_t t = a.team_ids[i];
+        const PTeam tm = a.team[t];
+        if (ld_agent(a.ctr + kCtrArrive + tm.dom) <= tm.slot0) run_whole<S>(a, t, smem);  // no member arrived
+      }
     if (tid == 0) {
       const uint32_t f = ld_agent(a.ctr + kCtrFail);
       if (f) __hip_atomic_fetch_or(a.status, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -331,6 +415,13 @@ __global__ __launch_bounds__(kBT) void k_batch_persist(PersistArgs a) {
     for (uint32_t i = tid; i < words; i += kBT)
       if (a.ctr + i != a.status) __hip_atomic_store(a.ctr + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+template <int S>
+__global__ __launch_bounds__(kBT) void k_batch_persist(PersistArgs a) {
+  extern __shared__ cx<float> smem[];
+  __shared__ uint32_t sh[6];
+  persist_body<S>(a, smem, sh);
 }
 
 template <int S>

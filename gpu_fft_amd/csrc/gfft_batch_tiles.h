@@ -151,23 +151,27 @@ __device__ __forceinline__ void run_tile(const BXform& x, int axis, bool inv, ui
 
 // ---------------------------------------------------------------------------
 // Persistent execution (gfft_batch_persist.hip): one launch runs the plan of
-// a mode (gfft_batch_sched.h), one workgroup per slot.  A variant set fixes
-// one compiled variant per class; the set's kernel calls the tile code of
-// those eight variants (its register count is the largest of them).
+// a mode (gfft_batch_sched.h).  A variant set fixes one compiled variant per
+// class; the set's kernel calls the tile code of those eight variants (its
+// register count is the largest of them).
 // ---------------------------------------------------------------------------
 constexpr int kSets = 2;
 // set 0: per class the variant with the fewest registers (kernel: 180
-// VGPRs); set 1: variant 1 of every class (228).  All variants 0 would need
-// 280 registers (with AGPRs), which does not launch.
+// VGPRs); set 1: variant 1 of every class (228).  Both run 2 waves per SIMD.
+// All variants 0 would need 280 registers (with AGPRs), which does not
+// launch.  (Set 0 held to 128 registers, 4 waves per SIMD, compiles with a
+// few spills and ran the team levels 2-4x slower.)
 constexpr int kSetVariant[kSets][kClasses] = {{2, 2, 1, 1, 0, 1, 1, 0}, {1, 1, 1, 1, 1, 1, 1, 1}};
 
 struct PersistArgs {
   const BXform* xf;            // descriptors, batch order (= team order)
   const PSeg* seg;             // levels of every transform
   const PTeam* team;           // one per transform
-  const PSlot* slot;           // one per workgroup
-  const PEntry* entry;         // what the slots run
-  uint32_t nslots;
+  const PEntry* local;         // domains * per_dom local slots, domain-major
+  const uint32_t* solo;        // solo queue (transforms run whole by one workgroup)
+  const uint32_t* team_ids;    // transforms run by teams
+  uint32_t domains, per_dom;   // L2 domains (XCDs) and local slots per domain
+  uint32_t nsolo, nteams;
   uint32_t claim_words;
   uint32_t* ctr;               // counter words (kCtrShares + claim_words)
   uint32_t* status;            // sticky status word the failure bits are ORed into

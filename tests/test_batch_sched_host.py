@@ -1,8 +1,9 @@
-"""Host-only check of the persistent batched kernel's work queue
-(gpu_fft_amd/csrc/gfft_batch_sched.h): the native test builds random
-queues, checks the dependency order, and simulates the kernel's ticket /
-claim / steal / wait / reset loop with threads under 1, 3, 8 and all resident
-workgroups (see tests/native/test_batch_sched.cpp)."""
+"""Host-only check of the persistent batched kernel's work plan
+(gpu_fft_amd/csrc/gfft_batch_sched.h): the native test builds random plans,
+checks their invariants, and simulates the kernel's arrival / claim / steal /
+wait / solo-queue / orphan / reset loop with threads under 1, 3, 8 and all
+resident workgroups, spread over the domains round robin, all in one domain
+or at random (see tests/native/test_batch_sched.cpp)."""
 import os
 import subprocess
 
