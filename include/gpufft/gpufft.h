@@ -260,7 +260,7 @@ int gfft_batch_exec(gfft_batch batch, int mode, gfft_stream_t stream);
  * of 65536 words that the library holds on each GPU until the process ends
  * (device address taken once per device with hipGetSymbolAddress; zeroed by a
  * fill kernel on the first call on a device).  Calls no allocating routine.
- * A plan needing more than 65532 share words returns
+ * A plan needing more than 65524 share words returns
  * GFFT_ERR_SCRATCH_TOO_SMALL.  Every call on one GPU shares the array:
  * calls that overlap on one GPU (other streams, other batches) race.
  * Failures are reported by gfft_batch_status like those of gfft_batch_exec. */
