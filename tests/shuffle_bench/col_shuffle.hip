@@ -21,10 +21,13 @@
 // instruction now touches 16 rows x 4 columns (32 B per row segment) instead
 // of 4 rows x 16 columns.
 //
-// Output: one JSON line with the library's x-pass time (tuned plan), the
-// variant's time and the largest relative difference between the two
-// results, for the 128^3 and 256^3 FP32 grids.  Exit status 1 if the results
-// differ by more than 1e-5 (relative L2 per grid) or a call fails.
+// A second form (k_cols_lane_lds) keeps the library's global access pattern
+// and moves the tile between it and the lane layout through LDS.
+//
+// Output: one JSON line with the library's x-pass time (tuned plan), both
+// variants' times and their relative L2 differences from the library's
+// result, for the 128^3 and 256^3 FP32 grids.  Exit status 1 if a result
+// differs by more than 1e-5 or a call fails.
 //
 // Build: make -C tests/shuffle_bench     Run: tests/shuffle_bench/col_shuffle [reps]
 #include <hip/hip_runtime.h>
@@ -89,6 +92,56 @@ __device__ __forceinline__ void dft16(cx<float> (&a)[16], const cx<float>* tab, 
   }
 }
 
+// The same lane FFT with the library's global access pattern: the tile is
+// loaded and stored columns-fastest (each wavefront instruction covers 4 rows
+// x 16 columns, 128 B row segments) and moved between that layout and the
+// lane layout through LDS (two LDS round trips, as many as the library's
+// three-stage Stockham schedules exchange through).
+template <int N1>
+__device__ __forceinline__ void lane_fft(cx<float> (&v)[N1], int t, const cx<float>* __restrict__ tab) {
+  constexpr int N = N1 * 16;
+  if constexpr (N1 == 8)
+    gfft::Dft<8, false, float>::run(v);
+  else
+    dft16(v, tab, N / 16);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) v[k1] = gfft::twmul<false>(v[k1], tab[t * k1]);
+  const cx<float> one = mk<float>(1.0f, 0.0f);
+  const cx<float> w8 = (t & 8) ? tab[(t & 7) * (N / 16)] : one;
+  const cx<float> w4 = (t & 4) ? tab[(t & 3) * (N / 8)] : one;
+  const cx<float> w2 = (t & 2) ? tab[(t & 1) * (N / 4)] : one;
+  lane_stage<8>(v, (t & 8) ? -1.0f : 1.0f, w8);
+  lane_stage<4>(v, (t & 4) ? -1.0f : 1.0f, w4);
+  lane_stage<2>(v, (t & 2) ? -1.0f : 1.0f, w2);
+  lane_stage<1>(v, (t & 1) ? -1.0f : 1.0f, one);
+}
+
+template <int N1>
+__global__ __launch_bounds__(256) void k_cols_lane_lds(cx<float>* data, int64_t stride,
+                                                       const cx<float>* __restrict__ tab) {
+  constexpr int N = N1 * 16, LP = 17;  // LDS row pitch (16 columns + 1)
+  __shared__ cx<float> sm[N * LP];
+  const int col = threadIdx.x & 15, r0 = threadIdx.x >> 4;  // columns-fastest I/O layout
+  const int lane = threadIdx.x & 63, t = lane & 15, cl = (threadIdx.x >> 6) * 4 + (lane >> 4);
+  cx<float>* d = data + (int64_t)blockIdx.x * 16;
+  cx<float> v[N1];
+#pragma unroll
+  for (int j = 0; j < N1; ++j) v[j] = d[(int64_t)(r0 + 16 * j) * stride + col];
+#pragma unroll
+  for (int j = 0; j < N1; ++j) sm[(r0 + 16 * j) * LP + col] = v[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < N1; ++j) v[j] = sm[(t + 16 * j) * LP + cl];
+  lane_fft<N1>(v, t, tab);
+  const int k2 = ((t & 1) << 3) | ((t & 2) << 1) | ((t & 4) >> 1) | ((t & 8) >> 3);
+  __syncthreads();  // every lane-layout read of sm is done
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) sm[(k1 + N1 * k2) * LP + cl] = v[k1];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < N1; ++j) d[(int64_t)(r0 + 16 * j) * stride + col] = sm[(r0 + 16 * j) * LP + col];
+}
+
 // tab[m] = W_N^m, m < N
 template <int N1>
 __global__ __launch_bounds__(256) void k_cols_lane(cx<float>* data, int64_t stride, const cx<float>* __restrict__ tab) {
@@ -98,21 +151,7 @@ __global__ __launch_bounds__(256) void k_cols_lane(cx<float>* data, int64_t stri
   cx<float> v[N1];
 #pragma unroll
   for (int j = 0; j < N1; ++j) v[j] = data[(int64_t)(t + 16 * j) * stride + c];
-  if constexpr (N1 == 8)
-    gfft::Dft<8, false, float>::run(v);
-  else
-    dft16(v, tab, N / 16);
-#pragma unroll
-  for (int k1 = 1; k1 < N1; ++k1) v[k1] = gfft::twmul<false>(v[k1], tab[t * k1]);
-  // per-lane stage twiddles: W_{2H}^(t mod H) on the upper lanes of each span
-  const cx<float> one = mk<float>(1.0f, 0.0f);
-  const cx<float> w8 = (t & 8) ? tab[(t & 7) * (N / 16)] : one;
-  const cx<float> w4 = (t & 4) ? tab[(t & 3) * (N / 8)] : one;
-  const cx<float> w2 = (t & 2) ? tab[(t & 1) * (N / 4)] : one;
-  lane_stage<8>(v, (t & 8) ? -1.0f : 1.0f, w8);
-  lane_stage<4>(v, (t & 4) ? -1.0f : 1.0f, w4);
-  lane_stage<2>(v, (t & 2) ? -1.0f : 1.0f, w2);
-  lane_stage<1>(v, (t & 1) ? -1.0f : 1.0f, one);
+  lane_fft<N1>(v, t, tab);
   const int k2 = ((t & 1) << 3) | ((t & 2) << 1) | ((t & 4) >> 1) | ((t & 8) >> 3);  // bitrev4(t)
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) data[(int64_t)(k1 + N1 * k2) * stride + c] = v[k1];
@@ -137,7 +176,7 @@ __global__ __launch_bounds__(256) void k_cols_lane(cx<float>* data, int64_t stri
 
 struct Result {
   int n;
-  double lib_us, lane_us, rel_err;
+  double lib_us, lane_us, lds_us, rel_err, rel_err_lds;
 };
 
 template <int N1>
@@ -158,9 +197,10 @@ Result run_grid(int reps) {
   }
   hipStream_t s;
   CK(hipStreamCreate(&s));
-  cx<float> *a = nullptr, *b = nullptr, *tab = nullptr;
+  cx<float> *a = nullptr, *b = nullptr, *c = nullptr, *tab = nullptr;
   CK(hipMalloc(&a, bytes));
   CK(hipMalloc(&b, bytes));
+  CK(hipMalloc(&c, bytes));
   CK(hipMalloc(&tab, sizeof(cx<float>) * N));
   CG(gfft_plan_tune(plan, a, s, 3));
   std::vector<cx<float>> h((size_t)elems), ht(N);
@@ -180,25 +220,30 @@ Result run_grid(int reps) {
   // correctness: the same input through both x passes
   CK(hipMemcpy(a, h.data(), sizeof(cx<float>) * elems, hipMemcpyHostToDevice));
   CK(hipMemcpy(b, h.data(), sizeof(cx<float>) * elems, hipMemcpyHostToDevice));
+  CK(hipMemcpy(c, h.data(), sizeof(cx<float>) * elems, hipMemcpyHostToDevice));
   CG(gfft_exec_axis(plan, 2, 0, a, s));
   const dim3 grid((unsigned)(ncols / 16)), block(256);
   hipLaunchKernelGGL(k_cols_lane<N1>, grid, block, 0, s, b, ncols, tab);
+  hipLaunchKernelGGL(k_cols_lane_lds<N1>, grid, block, 0, s, c, ncols, tab);
   CK(hipGetLastError());
   CK(hipStreamSynchronize(s));
-  std::vector<cx<float>> ra((size_t)elems), rb((size_t)elems);
+  std::vector<cx<float>> ra((size_t)elems), rb((size_t)elems), rc((size_t)elems);
   CK(hipMemcpy(ra.data(), a, sizeof(cx<float>) * elems, hipMemcpyDeviceToHost));
   CK(hipMemcpy(rb.data(), b, sizeof(cx<float>) * elems, hipMemcpyDeviceToHost));
-  double num = 0, den = 0;
+  CK(hipMemcpy(rc.data(), c, sizeof(cx<float>) * elems, hipMemcpyDeviceToHost));
+  double num = 0, num_c = 0, den = 0;
   for (int64_t i = 0; i < elems; ++i) {
     const double dx = (double)ra[i].x - rb[i].x, dy = (double)ra[i].y - rb[i].y;
+    const double ex = (double)ra[i].x - rc[i].x, ey = (double)ra[i].y - rc[i].y;
     num += dx * dx + dy * dy;
+    num_c += ex * ex + ey * ey;
     den += (double)ra[i].x * ra[i].x + (double)ra[i].y * ra[i].y;
   }
   // timing: the two passes alternate in blocks on one stream
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  double lib_ms = 0, lane_ms = 0;
+  double lib_ms = 0, lane_ms = 0, lds_ms = 0;
   for (int round = 0; round < 4; ++round) {
     for (int w = 0; w < 3; ++w) CG(gfft_exec_axis(plan, 2, 0, a, s));
     CK(hipEventRecord(e0, s));
@@ -216,12 +261,22 @@ Result run_grid(int reps) {
     CK(hipGetLastError());
     CK(hipEventElapsedTime(&ms, e0, e1));
     lane_ms += ms;
+    for (int w = 0; w < 3; ++w) hipLaunchKernelGGL(k_cols_lane_lds<N1>, grid, block, 0, s, c, ncols, tab);
+    CK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_cols_lane_lds<N1>, grid, block, 0, s, c, ncols, tab);
+    CK(hipEventRecord(e1, s));
+    CK(hipEventSynchronize(e1));
+    CK(hipGetLastError());
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    lds_ms += ms;
   }
-  Result res{N, 1e3 * lib_ms / (4.0 * reps), 1e3 * lane_ms / (4.0 * reps), std::sqrt(num / den)};
+  Result res{N, 1e3 * lib_ms / (4.0 * reps), 1e3 * lane_ms / (4.0 * reps), 1e3 * lds_ms / (4.0 * reps),
+             std::sqrt(num / den), std::sqrt(num_c / den)};
   CK(hipEventDestroy(e0));
   CK(hipEventDestroy(e1));
   CK(hipFree(a));
   CK(hipFree(b));
+  CK(hipFree(c));
   CK(hipFree(tab));
   CK(hipStreamDestroy(s));
   CG(gfft_plan_destroy(plan));
@@ -239,11 +294,12 @@ int main(int argc, char** argv) {
               reps);
   for (int i = 0; i < 2; ++i) {
     const double bytes = 2.0 * (double)r[i].n * r[i].n * (r[i].n / 2 + 1) * 8.0;
-    std::printf("%s{\"n\": %d, \"library_us\": %.2f, \"lane_shuffle_us\": %.2f, \"library_GBps\": %.1f, "
-                "\"lane_shuffle_GBps\": %.1f, \"rel_l2_diff\": %.3g}",
-                i ? ", " : "", r[i].n, r[i].lib_us, r[i].lane_us, bytes / (r[i].lib_us * 1e3),
-                bytes / (r[i].lane_us * 1e3), r[i].rel_err);
-    ok = ok && r[i].rel_err <= 1e-5;
+    std::printf("%s{\"n\": %d, \"library_us\": %.2f, \"lane_shuffle_us\": %.2f, \"lane_shuffle_lds_io_us\": %.2f, "
+                "\"library_GBps\": %.1f, \"lane_shuffle_GBps\": %.1f, \"lane_shuffle_lds_io_GBps\": %.1f, "
+                "\"rel_l2_diff\": %.3g, \"rel_l2_diff_lds_io\": %.3g}",
+                i ? ", " : "", r[i].n, r[i].lib_us, r[i].lane_us, r[i].lds_us, bytes / (r[i].lib_us * 1e3),
+                bytes / (r[i].lane_us * 1e3), bytes / (r[i].lds_us * 1e3), r[i].rel_err, r[i].rel_err_lds);
+    ok = ok && r[i].rel_err <= 1e-5 && r[i].rel_err_lds <= 1e-5;
   }
   std::printf("]}\n");
   return ok ? 0 : 1;
