@@ -56,6 +56,25 @@ __global__ __launch_bounds__(kBT) void k_batch_level(LevelArgs a) {
   }
 }
 
+// Fused z + y level of a plane class: one x-plane per tile, in LDS.
+template <int C, int V>
+__global__ __launch_bounds__(kBT) void k_batch_plane(LevelArgs a) {
+  using PP = BPlane<C, V>;
+  static_assert(PP::NT == kBT, "plane pass shape");
+  __shared__ cx<float> smem[PP::PLANE + PP::SCRATCH];
+  for (uint32_t k = blockIdx.x; k < a.total; k += gridDim.x) {
+    int lo = 0, hi = a.count;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.prefix[mid] <= k)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    plane_tile<PP>(a.xf[lo], a.inv != 0, k - a.prefix[lo], a.tw, smem);
+  }
+}
+
 using LaunchFn = hipError_t (*)(dim3, const LevelArgs&, hipStream_t);
 
 template <int C, int V>
@@ -64,18 +83,31 @@ hipError_t launch_cv(dim3 grid, const LevelArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int C, int V>
+hipError_t launch_plane_cv(dim3 grid, const LevelArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_batch_plane<C, V>), grid, dim3(kBT), 0, s, a);
+  return hipGetLastError();
+}
+
 struct VarInfo {
   Schedule sched;
   int P, W;
   const void* kernel;
   LaunchFn launch;
+  const void* plane_kernel;  // fused z + y level (plane classes), else null
+  LaunchFn plane_launch;
 };
 
 template <int C, int V>
 VarInfo var_info() {
   using F = BCfg<C, V>;
-  return VarInfo{make_sched<typename F::SC>(), F::P, F::W, reinterpret_cast<const void*>(&k_batch_level<C, V>),
-                 &launch_cv<C, V>};
+  VarInfo vi{make_sched<typename F::SC>(), F::P, F::W, reinterpret_cast<const void*>(&k_batch_level<C, V>),
+             &launch_cv<C, V>, nullptr, nullptr};
+  if constexpr (C < kPlaneClasses) {
+    vi.plane_kernel = reinterpret_cast<const void*>(&k_batch_plane<C, V>);
+    vi.plane_launch = &launch_plane_cv<C, V>;
+  }
+  return vi;
 }
 
 template <int C>
@@ -124,7 +156,8 @@ struct gfft_batch_s {
   size_t prefix_off[gfft::batch::kClasses][gfft::batch::kVariants][3] = {};
   uint32_t level_tiles[gfft::batch::kClasses][gfft::batch::kVariants][3][gfft::batch::kMaxLevels] = {};
   void* d_tw[gfft::batch::kClasses][gfft::batch::kVariants] = {};
-  int occ[gfft::batch::kClasses][gfft::batch::kVariants] = {};  // resident workgroups per CU
+  int occ[gfft::batch::kClasses][gfft::batch::kVariants] = {};        // resident workgroups per CU
+  int occ_plane[gfft::batch::kClasses][gfft::batch::kVariants] = {};  // the same for the plane levels
   float tuned_ms[2] = {-1.0f, -1.0f};  // last tune: ms per execution, persistent and levels
   int lane[gfft::batch::kClasses] = {};  // internal stream of the class's chain (eager path)
   int cus = 0;
@@ -161,9 +194,11 @@ struct gfft_batch_s {
 
 namespace {
 
-const int kModeAxis[3][gfft::batch::kMaxLevels] = {{0, 1, 2}, {2, 1, 0}, {0, 1, 2, 2, 1, 0}};
-const int kModeInv[3][gfft::batch::kMaxLevels] = {{0, 0, 0}, {1, 1, 1}, {0, 0, 0, 1, 1, 1}};
-const int kModeLevels[3] = {3, 3, 6};
+// Levels of each mode: R2C z, y, (z + y fused), x; C2R the reverse.  A
+// transform has either the z and y levels or the fused one (BXform::plane).
+const int kModeAxis[3][gfft::batch::kMaxLevels] = {{0, 1, 3, 2}, {2, 3, 1, 0}, {0, 1, 3, 2, 2, 3, 1, 0}};
+const int kModeInv[3][gfft::batch::kMaxLevels] = {{0, 0, 0, 0}, {1, 1, 1, 1}, {0, 0, 0, 0, 1, 1, 1, 1}};
+const int kModeLevels[3] = {4, 4, 8};
 // Shares per team member and level in the persistent plans: a member runs
 // its own in turn, so a slow member's later ones can be taken by the others.
 constexpr uint32_t kSharesPerMember = 2;
@@ -181,7 +216,8 @@ int device_xcc_count() {
 }
 
 uint32_t tiles_of(const BXform& x, int axis, int w) {
-  if (x.cls[axis] < 0) return 0;
+  if (gfft::batch::level_cls(x, axis) < 0) return 0;
+  if (axis == gfft::batch::kAxisPlane) return (uint32_t)(x.nrows / x.ny);  // one tile per x-plane
   if (axis == 0) return (uint32_t)(((x.nrows + 1) / 2 + w - 1) / w);
   const uint32_t nc = axis == 1 ? x.ncols_y : x.ncols_x;
   return (nc + w - 1) / w;
@@ -253,8 +289,9 @@ gfft::batch::LevelArgs level_args(const gfft_batch_s* b, int c, int mode, int l)
   return a;
 }
 
-uint32_t level_grid(const gfft_batch_s* b, int c, uint32_t total) {
-  return std::min<uint32_t>(total, (uint32_t)(b->cus * b->occ[c][b->var[c]]));
+uint32_t level_grid(const gfft_batch_s* b, int c, int axis, uint32_t total) {
+  const int occ = axis == gfft::batch::kAxisPlane ? b->occ_plane[c][b->var[c]] : b->occ[c][b->var[c]];
+  return std::min<uint32_t>(total, (uint32_t)(b->cus * occ));
 }
 
 // The plain variant: the class's transforms one after another through their
@@ -279,7 +316,9 @@ int launch_chain(const gfft_batch_s* b, int c, int mode, hipStream_t s) {
   for (int l = 0; l < kModeLevels[mode]; ++l) {
     const gfft::batch::LevelArgs a = level_args(b, c, mode, l);
     if (a.total == 0) continue;
-    if (gfft::var(c, b->var[c]).launch(dim3(level_grid(b, c, a.total)), a, s) != hipSuccess) return GFFT_ERR_HIP;
+    const gfft::VarInfo& vi = gfft::var(c, b->var[c]);
+    const gfft::LaunchFn fn = a.axis == gfft::batch::kAxisPlane ? vi.plane_launch : vi.launch;
+    if (!fn || fn(dim3(level_grid(b, c, a.axis, a.total)), a, s) != hipSuccess) return GFFT_ERR_HIP;
   }
   return GFFT_OK;
 }
@@ -348,11 +387,11 @@ int build_plans(gfft_batch_s* b) {
     for (int l = 0; l < kModeLevels[m]; ++l) lv[l] = PLevelSpec{kModeAxis[m][l], kModeInv[m][l]};
     auto tiles = [&](uint32_t t, int l) {
       const BXform& x = b->hx[t];
-      const int c = x.cls[kModeAxis[m][l]];
+      const int c = level_cls(x, kModeAxis[m][l]);
       return c < 0 ? 0u : tiles_of(x, kModeAxis[m][l], w[c]);
     };
     const PPlan pl = build_plan(
-        (uint32_t)b->count, kModeLevels[m], lv, tiles, [&](uint32_t t, int l) { return b->hx[t].cls[kModeAxis[m][l]]; },
+        (uint32_t)b->count, kModeLevels[m], lv, tiles, [&](uint32_t t, int l) { return level_cls(b->hx[t], kModeAxis[m][l]); },
         [&](uint32_t t) {
           const BXform& x = b->hx[t];
           double levels = 0;
@@ -492,8 +531,13 @@ int build_graph(gfft_batch_s* b, int mode, hipGraph_t* out) {
       if (a.total == 0) continue;
       void* params[] = {&a};
       hipKernelNodeParams p{};
-      p.func = const_cast<void*>(var(c, b->var[c]).kernel);
-      p.gridDim = dim3(level_grid(b, c, a.total));
+      const VarInfo& vi = var(c, b->var[c]);
+      p.func = const_cast<void*>(a.axis == kAxisPlane ? vi.plane_kernel : vi.kernel);
+      if (!p.func) {
+        (void)hipGraphDestroy(g);
+        return GFFT_ERR_HIP;
+      }
+      p.gridDim = dim3(level_grid(b, c, a.axis, a.total));
       p.blockDim = dim3(kBT);
       p.sharedMemBytes = 0;
       p.kernelParams = params;
@@ -659,6 +703,7 @@ int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec)
     x.cls[2] = nx > 1 ? class_of(nx) : -1;
     x.scale_r2c = 1.0f;
     x.scale_c2r = 1.0f;
+    x.plane = ny > 1 && x.cls[0] < kPlaneClasses ? 1 : 0;
     b->bytes[i] = (size_t)nx * ny * x.K * 8;
     b->n[x.cls[0]]++;
     work[x.cls[0]] += (double)b->bytes[i];
@@ -716,6 +761,12 @@ int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec)
           (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, var(c, v).kernel, kBT, 0) != hipSuccess || occ < 1))
         occ = 1;
       b->occ[c][v] = occ;
+      occ = 0;
+      if (st == GFFT_OK && var(c, v).plane_kernel &&
+          (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, var(c, v).plane_kernel, kBT, 0) != hipSuccess ||
+           occ < 1))
+        occ = 1;
+      b->occ_plane[c][v] = occ;
     }
   }
   for (int i = 0; i < kLanes && st == GFFT_OK; ++i) {

@@ -73,7 +73,16 @@ struct Seq {
   template <bool INV>
   using COLS = ColsPass<float, INV, SC, P, W, true>;
   static constexpr int kTw = twiddle_count<SC>();
-  static constexpr int lds_entries() { return F::lds_entries() + kTw; }
+  static constexpr bool kPlane = F::CLS < kPlaneClasses;  // z + y fused levels
+  static constexpr int base_entries() {
+    if constexpr (kPlane) {
+      using PP = BPlane<F::CLS, F::VAR>;
+      return F::lds_entries() > PP::PLANE + PP::SCRATCH ? F::lds_entries() : PP::PLANE + PP::SCRATCH;
+    } else {
+      return F::lds_entries();
+    }
+  }
+  static constexpr int lds_entries() { return base_entries() + kTw; }
 
   __device__ static void rows_r2c(const BXform& x, uint32_t first, uint32_t step, uint32_t count,
                                   const cx<float>* stw, cx<float>* smem) {
@@ -158,11 +167,16 @@ struct Seq {
   // its own kernel's register limit
   __device__ __forceinline__ static void run(const BXform& x, int axis, int inv, uint32_t first, uint32_t step,
                                              uint32_t count, const cx<float>* tw, cx<float>* smem) {
-    cx<float>* stw = smem + F::lds_entries();
+    cx<float>* stw = smem + base_entries();
     for (int i = threadIdx.x; i < SC::twoff(SC::S); i += kBT) stw[i] = tw[i];
     __syncthreads();  // the table is complete (and the previous call's LDS reads are done)
     if (count > 0) {
-      if (axis == 0) {
+      if (axis == kAxisPlane) {
+        if constexpr (kPlane) {
+          for (uint32_t k = 0; k < count; ++k)
+            plane_tile<BPlane<F::CLS, F::VAR>>(x, inv != 0, first + k * step, stw, smem);
+        }
+      } else if (axis == 0) {
         if (inv)
           rows_c2r(x, first, step, count, stw, smem);
         else

@@ -4,10 +4,14 @@
 // of a transform belong to one class); each class has kVariants compiled
 // variants (radix order; points per thread, which sets the pencils per
 // 256-thread tile) and every tile runs the same register-engine tile code as
-// the single-transform kernels.
+// the single-transform kernels.  For the classes up to 128 the z and y
+// passes of a transform run fused, one x-plane per tile held in LDS (the
+// plane kernels' code), so those transforms make two HBM round trips per
+// direction instead of three.
 #pragma once
 #include "gfft_batch_sched.h"
 #include "gfft_kernels.h"
+#include "gfft_plane.h"
 
 namespace gfft {
 namespace batch {
@@ -16,7 +20,9 @@ constexpr int kBT = 256;  // threads per workgroup, every tile kind
 constexpr int kClasses = 8;
 constexpr int kVariants = 3;
 constexpr int kPlainVariant = kVariants;  // per-transform plans
-constexpr int kMaxLevels = 6;
+constexpr int kMaxLevels = 8;
+constexpr int kAxisPlane = 3;    // level axis code of the fused z + y plane pass
+constexpr int kPlaneClasses = 5;  // classes 0 .. 4 (32 .. 128) fuse z and y
 constexpr int kLanes = 4;  // internal streams of the eager path
 
 // Variant V of size class C: radix order and points per thread P; with
@@ -60,6 +66,7 @@ constexpr int kClassN[kClasses] = {32, 48, 64, 96, 128, 160, 192, 256};
 
 template <int C, int V>
 struct BCfg {
+  static constexpr int CLS = C, VAR = V;
   using SC = typename BVar<C, V>::SC;
   static constexpr int N = SC::N;
   static_assert(N == kClassN[C], "variant size");
@@ -78,6 +85,37 @@ struct BCfg {
   }
 };
 
+// Shape of the fused plane pass of a class (256 threads): z and y points per
+// thread (the z exchange of a row pair stays inside one wavefront) and the
+// kz columns per y step.
+template <int C>
+struct PlaneShape;
+template <>
+struct PlaneShape<0> {
+  static constexpr int PZ = 4, PY = 4, CW = 32;
+};
+template <>
+struct PlaneShape<1> {
+  static constexpr int PZ = 6, PY = 3, CW = 16;
+};
+template <>
+struct PlaneShape<2> {
+  static constexpr int PZ = 8, PY = 4, CW = 16;
+};
+template <>
+struct PlaneShape<3> {
+  static constexpr int PZ = 12, PY = 6, CW = 16;
+};
+template <>
+struct PlaneShape<4> {
+  static constexpr int PZ = 8, PY = 8, CW = 16;
+};
+
+// The plane pass of class C with variant V's radix schedule (and twiddles).
+template <int C, int V>
+using BPlane = PlanePass<float, typename BVar<C, V>::SC, PlaneShape<C>::PZ, typename BVar<C, V>::SC,
+                         PlaneShape<C>::PY, PlaneShape<C>::CW>;
+
 inline int class_of(int64_t n) {
   for (int c = 0; c < kClasses; ++c)
     if (kClassN[c] == n) return c;
@@ -94,7 +132,15 @@ struct BXform {
   uint32_t ncols_y;  // Nx * K
   uint32_t ncols_x;  // Ny * K
   float scale_r2c, scale_c2r;
+  int32_t plane;     // 1: the z and y passes run fused per x-plane (level axis kAxisPlane)
 };
+
+// Size class of a level of transform x (-1: the level is empty for x).
+GFFT_HD inline int level_cls(const BXform& x, int axis) {
+  if (axis == kAxisPlane) return x.plane ? x.cls[0] : -1;
+  if (x.plane && axis < 2) return -1;
+  return x.cls[axis];
+}
 
 // One level launch: one axis pass over every transform of one class.
 struct LevelArgs {
@@ -102,7 +148,7 @@ struct LevelArgs {
   const uint32_t* prefix;  // [count + 1] tile prefix sums of this level
   const cx<float>* tw;     // twiddle table of the class's variant
   int32_t count;
-  int32_t axis;            // 0 = z rows, 1 = y columns, 2 = x columns
+  int32_t axis;            // 0 = z rows, 1 = y columns, 2 = x columns, 3 = z + y per x-plane
   int32_t inv;
   uint32_t total;          // tiles of this level = prefix[count]
 };
@@ -147,6 +193,38 @@ __device__ __forceinline__ void run_tile(const BXform& x, int axis, bool inv, ui
     F::INV::tile(d, d, g, tw, tile, smem);
   else
     F::FWD::tile(d, d, g, tw, tile, smem);
+}
+
+// x-plane `plane` of transform x through the fused plane pass PP (R2C: z
+// then y; C2R: y then z, writing the Nz reals of every row).  smem holds
+// PP::PLANE + PP::SCRATCH entries; ends with a barrier, so the next call may
+// reuse them.
+template <typename PP>
+__device__ __forceinline__ void plane_tile(const BXform& x, bool inv, uint32_t plane, const cx<float>* tw,
+                                           cx<float>* smem) {
+  cx<float>* g = reinterpret_cast<cx<float>*>(x.data) + (size_t)plane * PP::PLANE;
+  PP::copy_in(smem, g);
+  __syncthreads();
+  if (!inv) {
+    PP::z_r2c(smem, tw);
+    __syncthreads();
+    // the forward scale belongs to this pass when x is length 1
+    const float sc = x.cls[2] < 0 ? x.scale_r2c : 1.0f;
+    PP::template y_pass<false>(smem, smem + PP::PLANE, tw, sc, sc != 1.0f);
+    __syncthreads();
+    for (int i = threadIdx.x; i < PP::PLANE; i += PP::NT) st_stream(g + i, smem[i]);
+  } else {
+    PP::template y_pass<true>(smem, smem + PP::PLANE, tw, 1.0f, false);
+    __syncthreads();
+    PP::z_c2r(smem, tw, x.scale_c2r, x.scale_c2r != 1.0f);
+    __syncthreads();
+    constexpr int H = PP::NZ / 2;
+    for (int i = threadIdx.x; i < PP::NY * H; i += PP::NT) {
+      const int y = i / H, m = i - y * H;
+      st_stream(g + y * PP::K + m, smem[y * PP::K + m]);
+    }
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------

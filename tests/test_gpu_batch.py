@@ -151,14 +151,14 @@ def test_tile_and_launch_counts():
         assert b.tiles(G.ROUND_TRIP)[1] == 1  # one persistent launch
         b.set_scheduler(G.Batch.LEVELS)
         tiles, launches = b.tiles(G.R2C)
-        # class 32: P=8 -> 4 threads/row -> 64 pairs or columns per tile
-        # class 64: P=8 -> 8 threads/row -> 32 pairs or columns per tile
-        z = -(-(32 * 32 // 2) // 64) + -(-(64 * 64 // 2) // 32) + -(-(64 // 2) // 32)
-        y = -(-(32 * 17) // 64) + -(-(64 * 33) // 32) + -(-(1 * 33) // 32)
+        # classes up to 128 run z and y fused: one tile per x-plane
+        planes = 64 + 1 + 32
+        # x columns, class 32: P=8 -> 4 threads per column -> 64 columns per
+        # tile; class 64: P=8 -> 8 threads -> 32 columns (length-1 x: none)
         x = -(-(32 * 17) // 64) + -(-(64 * 33) // 32)
-        assert tiles == z + y + x
-        assert launches == 6
-        assert b.tiles(G.ROUND_TRIP) == (2 * tiles, 12)
+        assert tiles == planes + x
+        assert launches == 4  # per class: the plane level and the x level
+        assert b.tiles(G.ROUND_TRIP) == (2 * tiles, 8)
 
 
 def test_batch_rejects_unsupported_sizes():
