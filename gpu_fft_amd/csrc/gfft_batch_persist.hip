@@ -19,9 +19,13 @@
 //             one atomic store of "done" to the share's word;
 //   wait:     the first wavefront reads the level's share words (atomic
 //             loads, 64 per instruction) until all are done; barrier; every
-//             wavefront invalidates the CU's L1 (buffer_inv sc0) before it
-//             loads the next level.
-// No L2 write-back or invalidation happens inside the launch.  While
+//             wavefront runs an agent-scope acquire fence (buffer_inv sc1:
+//             drops the CU's L1 lines, keeps the L2's dirty lines) before
+//             it loads the next level.
+// No L2 write-back happens inside the launch.  (A workgroup-scope L1
+// invalidate, buffer_inv sc0, is not enough outside threadgroup-split mode:
+// with it, lines a member had read in an earlier level were now and then
+// served stale from its L1.)  While
 // waiting, after steal_ticks, a member claims and runs any share still free
 // (its owner has not started), so a team never depends on all of its members
 // being resident.  Every wait is bounded: it gives up when the launch's
@@ -232,8 +236,8 @@ __device__ __forceinline__ uint32_t xcc_id() {
 // This wavefront's global stores have completed (they are in the L2).
 __device__ __forceinline__ void stores_complete() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// Later loads of this CU miss the L1 (they see what the L2 holds).
-__device__ __forceinline__ void invalidate_l1() { __asm__ volatile("buffer_inv sc0" ::: "memory"); }
+// Later loads of this wavefront see what the L2 holds (the L1 is dropped).
+__device__ __forceinline__ void acquire_level() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 // share m of level sg (tiles m, m + n, ... of n shares), then publish it as done
 template <int S>
@@ -393,7 +397,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* sm
         }
         if (st < ns) run_share<S>(a, x, sg, ns, st, smem);  // a stolen share
       }
-      if (!failed) invalidate_l1();  // the level's stores, from the L2
+      if (!failed) acquire_level();  // the level's stores, from the L2
     }
   }
   // solo transforms from the shared queue

@@ -416,3 +416,27 @@ def test_persistent_zero_wait_bound_drains():
         for (nx, ny, nz), buf, fp in zip(dims, work, host):
             got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
             assert O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), nz)) <= 1e-6
+
+
+def test_single_transform_teams_repeated_match_oracle():
+    """A single 48^3 or 64^3 transform runs as one large team (19-32 members
+    in one XCD) whose levels hand data over through the L2; each member must
+    read the previous level's data, not lines it kept in its L1.  Repeated
+    round trips against the FP64 oracle (a stale read left errors of 3e-2 to
+    1.6e-1 in about one run in three)."""
+    import torch
+    for n in (48, 64):
+        dims = [(n, n, n)]
+        say(f"case single-transform team {n}^3")
+        for rep in range(4):
+            bufs, host = inputs(torch, dims, 500 + 10 * n + rep)
+            with G.Batch(dims) as b:
+                assert b.scheduler() == G.Batch.PERSISTENT
+                b.set_scales(c2r=[1.0 / n ** 3])
+                b.set_data(bufs)
+                assert b.plan_info(G.ROUND_TRIP)["teams"] == 1
+                b.exec(G.ROUND_TRIP)
+                torch.cuda.synchronize()
+                assert b.status() == 0
+            err = O.rel_l2(bufs[0].cpu().numpy()[:, :, :n], host[0][:, :, :n])
+            assert err <= 1e-6, (n, rep, err)
