@@ -215,8 +215,8 @@ int device_xcc_count() {
   return n >= 1 && n <= (int)gfft::kMaxDomains ? n : 1;
 }
 
-uint32_t tiles_of(const BXform& x, int axis, int w) {
-  if (gfft::batch::level_cls(x, axis) < 0) return 0;
+uint32_t tiles_of(const BXform& x, int axis, int w, int max_plane = gfft::batch::kPlaneClasses) {
+  if (gfft::batch::level_cls(x, axis, max_plane) < 0) return 0;
   if (axis == gfft::batch::kAxisPlane) return (uint32_t)(x.nrows / x.ny);  // one tile per x-plane
   if (axis == 0) return (uint32_t)(((x.nrows + 1) / 2 + w - 1) / w);
   const uint32_t nc = axis == 1 ? x.ncols_y : x.ncols_x;
@@ -387,11 +387,12 @@ int build_plans(gfft_batch_s* b) {
     for (int l = 0; l < kModeLevels[m]; ++l) lv[l] = PLevelSpec{kModeAxis[m][l], kModeInv[m][l]};
     auto tiles = [&](uint32_t t, int l) {
       const BXform& x = b->hx[t];
-      const int c = level_cls(x, kModeAxis[m][l]);
-      return c < 0 ? 0u : tiles_of(x, kModeAxis[m][l], w[c]);
+      const int c = level_cls(x, kModeAxis[m][l], kPersistPlaneClasses);
+      return c < 0 ? 0u : tiles_of(x, kModeAxis[m][l], w[c], kPersistPlaneClasses);
     };
     const PPlan pl = build_plan(
-        (uint32_t)b->count, kModeLevels[m], lv, tiles, [&](uint32_t t, int l) { return level_cls(b->hx[t], kModeAxis[m][l]); },
+        (uint32_t)b->count, kModeLevels[m], lv, tiles,
+        [&](uint32_t t, int l) { return level_cls(b->hx[t], kModeAxis[m][l], kPersistPlaneClasses); },
         [&](uint32_t t) {
           const BXform& x = b->hx[t];
           double levels = 0;

@@ -77,7 +77,7 @@ struct Seq {
   template <bool INV>
   using COLS = ColsPass<float, INV, SC, P, W, true>;
   static constexpr int kTw = twiddle_count<SC>();
-  static constexpr bool kPlane = F::CLS < kPlaneClasses;  // z + y fused levels
+  static constexpr bool kPlane = F::CLS < kPersistPlaneClasses;  // z + y fused levels
   static constexpr int base_entries() {
     if constexpr (kPlane) {
       using PP = BPlane<F::CLS, F::VAR>;

@@ -22,7 +22,9 @@ constexpr int kVariants = 3;
 constexpr int kPlainVariant = kVariants;  // per-transform plans
 constexpr int kMaxLevels = 8;
 constexpr int kAxisPlane = 3;    // level axis code of the fused z + y plane pass
-constexpr int kPlaneClasses = 5;  // classes 0 .. 4 (32 .. 128) fuse z and y
+constexpr int kPlaneClasses = 7;         // level scheduler: classes 0 .. 6 (32 .. 192) fuse z and y
+constexpr int kPersistPlaneClasses = 5;  // persistent kernel: 0 .. 4 (a 160 or 192 plane needs
+                                         // 105-150 KB of LDS, one workgroup per CU)
 constexpr int kLanes = 4;  // internal streams of the eager path
 
 // Variant V of size class C: radix order and points per thread P; with
@@ -110,6 +112,14 @@ template <>
 struct PlaneShape<4> {
   static constexpr int PZ = 8, PY = 8, CW = 16;
 };
+template <>
+struct PlaneShape<5> {
+  static constexpr int PZ = 10, PY = 10, CW = 16;
+};
+template <>
+struct PlaneShape<6> {
+  static constexpr int PZ = 12, PY = 12, CW = 16;
+};
 
 // The plane pass of class C with variant V's radix schedule (and twiddles).
 template <int C, int V>
@@ -135,10 +145,12 @@ struct BXform {
   int32_t plane;     // 1: the z and y passes run fused per x-plane (level axis kAxisPlane)
 };
 
-// Size class of a level of transform x (-1: the level is empty for x).
-GFFT_HD inline int level_cls(const BXform& x, int axis) {
-  if (axis == kAxisPlane) return x.plane ? x.cls[0] : -1;
-  if (x.plane && axis < 2) return -1;
+// Size class of a level of transform x (-1: the level is empty for x);
+// z and y run fused when x.plane is set and the class is below max_plane.
+GFFT_HD inline int level_cls(const BXform& x, int axis, int max_plane = kPlaneClasses) {
+  const bool fused = x.plane && x.cls[0] < max_plane;
+  if (axis == kAxisPlane) return fused ? x.cls[0] : -1;
+  if (fused && axis < 2) return -1;
   return x.cls[axis];
 }
 
