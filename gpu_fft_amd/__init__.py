@@ -90,6 +90,10 @@ def lib() -> ctypes.CDLL:
     L.gfft_plan_set_chunking.argtypes = [vp, ctypes.c_int64]
     L.gfft_plan_set_walk.restype = ctypes.c_int
     L.gfft_plan_set_walk.argtypes = [vp, ctypes.c_int]
+    L.gfft_plan_set_walk_mask.restype = ctypes.c_int
+    L.gfft_plan_set_walk_mask.argtypes = [vp, ctypes.c_int]
+    L.gfft_plan_set_pass_order.restype = ctypes.c_int
+    L.gfft_plan_set_pass_order.argtypes = [vp, ctypes.c_int]
     L.gfft_plan_set_scale.restype = ctypes.c_int
     L.gfft_plan_set_scale.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.gfft_exec_axis.restype = ctypes.c_int
@@ -250,6 +254,17 @@ class Plan:
         """0: every pass walks its tiles in order; 1: passes alternate the
         order so each starts where the previous one ended (cache reuse)."""
         _check(lib().gfft_plan_set_walk(self.handle, int(mode)), "gfft_plan_set_walk")
+
+    def set_walk_mask(self, mask: int) -> None:
+        """Per-pass tile order: bit 3*direction + axis set = that pass walks
+        its tiles last to first (direction 0 = R2C, 1 = C2R; axis 0 = z,
+        1 = y, 2 = x)."""
+        _check(lib().gfft_plan_set_walk_mask(self.handle, int(mask)), "gfft_plan_set_walk_mask")
+
+    def set_pass_order(self, order: int) -> None:
+        """Strided pass order: bit 0 = R2C runs x before y, bit 1 = C2R runs
+        y before x (0 = the reference's order)."""
+        _check(lib().gfft_plan_set_pass_order(self.handle, int(order)), "gfft_plan_set_pass_order")
 
     def set_scale(self, r2c_scale: float = 1.0, c2r_scale: float = 1.0) -> None:
         """Output scales fused into the last pass of each direction."""

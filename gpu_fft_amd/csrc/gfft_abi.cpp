@@ -188,7 +188,19 @@ int gfft_plan_set_fused(gfft_plan plan, int on) {
 
 int gfft_plan_set_walk(gfft_plan plan, int mode) {
   if (!plan || mode < 0 || mode > 1) return GFFT_ERR_INVALID_ARG;
-  plan->core.walk = mode;
+  plan->core.walk_mask = mode ? 42 : 0;
+  return GFFT_OK;
+}
+
+int gfft_plan_set_walk_mask(gfft_plan plan, int mask) {
+  if (!plan || mask < 0 || mask > 63) return GFFT_ERR_INVALID_ARG;
+  plan->core.walk_mask = mask;
+  return GFFT_OK;
+}
+
+int gfft_plan_set_pass_order(gfft_plan plan, int order) {
+  if (!plan || order < 0 || order > 3) return GFFT_ERR_INVALID_ARG;
+  plan->core.pass_order = order;
   return GFFT_OK;
 }
 

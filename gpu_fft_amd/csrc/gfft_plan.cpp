@@ -457,12 +457,12 @@ double axis_scale(const PlanCore& c, int axis, bool inverse) {
   return axis == last ? c.r2c_scale : 1.0;
 }
 
-// Walk direction of a pass.  With alternating walks the passes of a round
-// trip run R2C z forward, y reversed, x forward; C2R x reversed, y forward,
-// z reversed: each pass starts on the tiles the previous pass wrote last,
-// which are still in the memory-side cache.
+// Walk direction of a pass.  With alternating walks (the default mask) the
+// passes of a round trip run R2C z forward, y reversed, x forward; C2R x
+// reversed, y forward, z reversed: each pass starts on the tiles the
+// previous pass wrote last, which are still in the memory-side cache.
 bool reversed_walk(const PlanCore& c, int axis, bool inverse) {
-  return c.walk == 1 && (inverse ? axis != 1 : axis == 1);
+  return (c.walk_mask >> ((inverse ? 3 : 0) + axis)) & 1;
 }
 
 bool use_plane(const PlanCore& c) { return c.fused && c.plane && c.d_twz_plane && c.d_twy_plane; }
@@ -847,7 +847,8 @@ int exec_r2c(const PlanCore& c, void* data, hipStream_t s) {
     if (st != GFFT_OK) return st;
     return exec_axis(c, 2, false, data, s);
   }
-  for (int axis = 0; axis < 3; ++axis) {
+  const int order[3] = {0, (c.pass_order & 1) ? 2 : 1, (c.pass_order & 1) ? 1 : 2};
+  for (int axis : order) {
     const int st = exec_axis(c, axis, false, data, s);
     if (st != GFFT_OK) return st;
   }
@@ -865,7 +866,8 @@ int exec_c2r(const PlanCore& c, void* data, hipStream_t s) {
     if (st != GFFT_OK) return st;
     return exec_zy_chunked(c, true, data, s);
   }
-  for (int axis = 2; axis >= 0; --axis) {
+  const int order[3] = {(c.pass_order & 2) ? 1 : 2, (c.pass_order & 2) ? 2 : 1, 0};
+  for (int axis : order) {
     const int st = exec_axis(c, axis, true, data, s);
     if (st != GFFT_OK) return st;
   }

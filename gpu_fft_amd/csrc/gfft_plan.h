@@ -36,7 +36,12 @@ struct PlanCore {
   int procs = 1, rank = 0, prec = 0;
   double r2c_scale = 1.0, c2r_scale = 1.0;  // fused output scales (1 = unnormalised)
   int64_t chunk_planes = 0;  // > 0: z and y passes alternate per block of x-planes
-  int walk = 1;              // 1: passes alternate their tile order (see exec_axis)
+  // Reversed tile walks, bit (inverse * 3 + axis): the default (42) is the
+  // alternating rule of gfft_plan_set_walk mode 1 (R2C y, C2R z and x)
+  int walk_mask = 42;
+  // Order of the strided passes: bit 0 = R2C runs x before y, bit 1 = C2R
+  // runs y before x (the z pass is always the first R2C / last C2R pass)
+  int pass_order = 0;
   AxisPlan z, y, x;
   // fused z + y kernels over LDS-resident x-planes (null: none compiled for
   // this geometry); `fused` selects them for full transforms

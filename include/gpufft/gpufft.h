@@ -157,6 +157,21 @@ int gfft_plan_set_chunking(gfft_plan plan, int64_t planes);
  * memory-side cache.  Bitwise the same results either way. */
 int gfft_plan_set_walk(gfft_plan plan, int mode);
 
+/* Per-pass tile order: bit (3 * direction + axis) set = that pass walks its
+ * tiles last to first (direction 0 = R2C, 1 = C2R; axis 0 = z, 1 = y,
+ * 2 = x), mask in [0, 63].  gfft_plan_set_walk(plan, 1) is mask 42 and
+ * mode 0 is mask 0.  Bitwise the same results for every mask.  No reference
+ * counterpart (tile order is internal to its library calls). */
+int gfft_plan_set_walk_mask(gfft_plan plan, int mask);
+
+/* Order of the two strided passes of full transforms: bit 0 set = the R2C
+ * runs x before y (z, x, y), bit 1 set = the C2R runs y before x (y, x, z);
+ * 0 (the default) is the reference's order (FFT_R2C.cu:45-69 z+y then x,
+ * FFT_C2R.cu:45-69 x then y+z).  The axes are separable, so every order
+ * computes the same transform; results agree to rounding.  The local
+ * halves of a slab exchange keep the reference's order. */
+int gfft_plan_set_pass_order(gfft_plan plan, int order);
+
 /* Persistent autotuning: the process-wide cache that gfft_plan_tune fills
  * and consults, as text, one line per tuned geometry:
  *   "<gfx arch>/<CUs>|<pass geometry> <chosen kernel description>"
