@@ -1,7 +1,6 @@
 // Compiled register-engine variants, FP32.  Column tiles are 16 columns wide
 // (16 x 8 B = 128 B per row segment) except where the tile would not fit LDS.
 #include "gfft_schedules.h"
-#include "gfft_lines.h"
 
 namespace gfft {
 
@@ -114,9 +113,6 @@ int registry_f32(const KernelEntry** out) {
       GFFT_ROWS_PIPE(float, Sc512, 8, 4, 4, 1),
       GFFT_ROWS_PIPE(float, Sc512, 8, 8, 2, 1),
       GFFT_ROWS_PIPE(float, Sc512, 16, 8, 2, 1),
-      // line-aligned chunk walk (gfft_lines.h)
-      GFFT_COLS_LINES(float, Sc512, 16, 16, 1),
-      GFFT_COLS_LINES(float, Sc512, 8, 16, 1),
   };
   *out = table;
   return (int)(sizeof(table) / sizeof(table[0]));

@@ -1,7 +1,6 @@
 // Compiled register-engine variants, FP64.  Column tiles are 8 columns wide
 // (8 x 16 B = 128 B per row segment) except where the tile would not fit LDS.
 #include "gfft_schedules.h"
-#include "gfft_lines.h"
 
 namespace gfft {
 
@@ -50,9 +49,6 @@ int registry_f64(const KernelEntry** out) {
       GFFT_COLS_PIPE_RA(double, Sc1024, 16, 8, 1, 1),
       GFFT_COLS_PIPE_RA(double, Sc1024, 16, 4, 2, 0),
       GFFT_COLS_PIPE_RA(double, Sc1024, 16, 4, 2, 1),
-      // line-aligned chunk walk (gfft_lines.h)
-      GFFT_COLS_LINES(double, Sc512, 16, 8, 1),
-      GFFT_COLS_LINES(double, Sc512, 8, 8, 1),
   };
   *out = table;
   return (int)(sizeof(table) / sizeof(table[0]));
