@@ -452,7 +452,9 @@ class Batch:
 
     def set_persist(self, set_index: int, team_factor: int) -> None:
         """Persistent scheduler: variant set (0..1) and team factor (a
-        transform gets a team of workgroups above that many slot shares)."""
+        transform gets a team of workgroups in one XCD above that many slot
+        shares; 0 = the global-level plan, every workgroup in every
+        transform's axis passes)."""
         _check(lib().gfft_batch_set_persist(self.handle, int(set_index), int(team_factor)), "gfft_batch_set_persist")
 
     def persist(self) -> dict:

@@ -172,7 +172,8 @@ struct gfft_batch_s {
   std::vector<const gfft::PlanCore*> xplan;  // per sorted position
   // persistent execution (default): one launch per mode following a plan
   int sched = GFFT_BATCH_PERSISTENT;
-  int pset = 0, pteam = 2;            // variant set, team factor (a team above pteam slot shares of work)
+  int pset = 0, pteam = 2;            // variant set, team factor (a team above pteam slot shares of
+                                      // work; 0: global-level plans, every workgroup in every transform)
   struct DevPlan {
     gfft::PSeg* seg = nullptr;
     gfft::PTeam* team = nullptr;
@@ -181,6 +182,13 @@ struct gfft_batch_s {
     uint32_t* team_ids = nullptr;
     uint32_t nslots = 0;  // workgroups of the launch (domains * per_dom; 0: nothing to run)
     uint32_t domains = 1, per_dom = 1, nsolo = 0, claim_words = 0, tiles = 0, teams_k2 = 0, members = 0;
+    // global-level plan (team factor 0)
+    gfft::WShare* wshare = nullptr;
+    uint32_t* lvl_off = nullptr;
+    uint32_t* own_off = nullptr;
+    uint32_t* own = nullptr;
+    uint32_t nlevels = 0, nshares = 0, level_base = 0;
+    size_t ctr_words = 0;  // counter words the launch uses
   } plan[3];
   uint32_t* d_ctr = nullptr;          // counter words, sized for the largest plan
   size_t ctr_words = 0;
@@ -202,6 +210,11 @@ const int kModeLevels[3] = {4, 4, 8};
 // Shares per team member and level in the persistent plans: a member runs
 // its own in turn, so a slow member's later ones can be taken by the others.
 constexpr uint32_t kSharesPerMember = 2;
+// Global-level plans: about three shares per slot and level (the level's
+// tail is then short), and the transforms below a quarter of a slot share of work in
+// the solo queue, run whole while workgroups wait at level hand-overs.
+constexpr uint32_t kWideSharesPerSlot = 3;
+constexpr double kWideSoloFactor = 0.25;
 
 // L2 domains (XCDs) of the current device; 1 where the count is unknown or
 // above what a plan holds (the kernel then gives team roles only to the
@@ -252,13 +265,17 @@ void free_plans(gfft_batch_s* b) {
     if (p.local) (void)hipFree(p.local);
     if (p.solo) (void)hipFree(p.solo);
     if (p.team_ids) (void)hipFree(p.team_ids);
+    if (p.wshare) (void)hipFree(p.wshare);
+    if (p.lvl_off) (void)hipFree(p.lvl_off);
+    if (p.own_off) (void)hipFree(p.own_off);
+    if (p.own) (void)hipFree(p.own);
     p = gfft_batch_s::DevPlan{};
   }
 }
 
 void release(gfft_batch_s* b) {
   drop_graphs(b);
-  if (b->d_ctr || b->plan[0].seg || b->plan[1].seg || b->plan[2].seg) (void)hipDeviceSynchronize();
+  if (b->d_ctr || b->plan[0].seg || b->plan[1].seg || b->plan[2].seg) (void)hipDeviceSynchronize();  // launches read them
   free_plans(b);
   if (b->d_ctr) (void)hipFree(b->d_ctr);
   if (b->h_abort) (void)hipHostFree(b->h_abort);
@@ -390,6 +407,49 @@ int build_plans(gfft_batch_s* b) {
       const int c = level_cls(x, kModeAxis[m][l], kPersistPlaneClasses);
       return c < 0 ? 0u : tiles_of(x, kModeAxis[m][l], w[c], kPersistPlaneClasses);
     };
+    gfft_batch_s::DevPlan& dp = b->plan[m];
+    if (b->pteam == 0) {
+      // relative cost of one tile: the complex values it moves
+      auto cost = [&](uint32_t t, int l) {
+        const BXform& x = b->hx[t];
+        const int axis = kModeAxis[m][l];
+        const int c = level_cls(x, axis, kPersistPlaneClasses);
+        if (c < 0) return 0.0;
+        if (axis == kAxisPlane) return (double)x.ny * x.K;
+        if (axis == 0) return 2.0 * w[c] * x.K;
+        return (double)w[c] * (axis == 1 ? x.ny : x.nrows / x.ny);
+      };
+      const WPlan wp = build_wide_plan(
+          (uint32_t)b->count, kModeLevels[m], lv, tiles,
+          [&](uint32_t t, int l) { return level_cls(b->hx[t], kModeAxis[m][l], kPersistPlaneClasses); }, cost,
+          domains * per_dom, kWideSharesPerSlot, kWideSoloFactor);
+      if (check_wide_plan(wp) != 0) return GFFT_ERR_INVALID_ARG;  // cannot happen: build_wide_plan's construction
+      std::vector<PTeam> xt(b->count);  // each transform's segs, for the solo transforms' whole runs
+      for (int t = 0; t < b->count; ++t) xt[t] = PTeam{wp.xseg0[t], wp.xnseg[t], 1u, 1u, 0u, 0u};
+      int st = upload_vec(&dp.seg, wp.segs);
+      if (st == GFFT_OK) st = upload_vec(&dp.team, xt);
+      if (st == GFFT_OK) st = upload_vec(&dp.solo, wp.solo);
+      if (st == GFFT_OK) st = upload_vec(&dp.wshare, wp.shares);
+      if (st == GFFT_OK) st = upload_vec(&dp.lvl_off, wp.lvl_off);
+      if (st == GFFT_OK) st = upload_vec(&dp.own_off, wp.own_off);
+      if (st == GFFT_OK) st = upload_vec(&dp.own, wp.own);
+      if (st != GFFT_OK) {
+        free_plans(b);
+        return st;
+      }
+      dp.nslots = wp.segs.empty() ? 0u : domains * per_dom;
+      dp.domains = domains;
+      dp.per_dom = per_dom;
+      dp.nlevels = wp.nlevels;
+      dp.nsolo = (uint32_t)wp.solo.size();
+      dp.nshares = (uint32_t)wp.shares.size();
+      dp.level_base = (uint32_t)wide_level_base(wp);
+      dp.members = dp.nslots;
+      for (const PSeg& sg : wp.segs) dp.tiles += sg.ntiles;
+      dp.ctr_words = wide_counter_words(wp);
+      need = std::max(need, dp.ctr_words);
+      continue;
+    }
     const PPlan pl = build_plan(
         (uint32_t)b->count, kModeLevels[m], lv, tiles,
         [&](uint32_t t, int l) { return level_cls(b->hx[t], kModeAxis[m][l], kPersistPlaneClasses); },
@@ -401,7 +461,6 @@ int build_plans(gfft_batch_s* b) {
         },
         domains, per_dom, (double)b->pteam, 2, kSharesPerMember);
     if (check_plan(pl) != 0) return GFFT_ERR_INVALID_ARG;  // cannot happen: build_plan's construction
-    gfft_batch_s::DevPlan& dp = b->plan[m];
     int st = upload_vec(&dp.seg, pl.segs);
     if (st == GFFT_OK) st = upload_vec(&dp.team, pl.teams);
     if (st == GFFT_OK) st = upload_vec(&dp.local, pl.local);
@@ -422,7 +481,8 @@ int build_plans(gfft_batch_s* b) {
         ++dp.teams_k2;
         dp.members += tm.k;
       }
-    need = std::max(need, counter_words(pl));
+    dp.ctr_words = counter_words(pl);
+    need = std::max(need, dp.ctr_words);
   }
   if (need > b->ctr_words) {  // counters for the largest plan, zeroed
     if (b->d_ctr) (void)hipFree(b->d_ctr);
@@ -459,9 +519,21 @@ gfft::batch::PersistArgs persist_args(const gfft_batch_s* b, int mode) {
   a.host_abort = b->h_abort;
   const double ticks = b->wait_limit_s * (double)b->wall_khz * 1e3;
   a.wait_ticks = ticks >= 1.8e19 ? ~0ull : (uint64_t)ticks;
-  a.steal_ticks = 0;  // a member steals as soon as its own shares are done
+  // team plans: a member steals as soon as its own shares are done; global-
+  // level plans: only after 50 us without completion (a list is unclaimed
+  // only while its slot's workgroup has not arrived)
+  a.steal_ticks = b->pteam == 0 ? (uint64_t)(50e-6 * (double)b->wall_khz * 1e3) : 0;
   for (int c = 0; c < kClasses; ++c)
     a.tw[c] = static_cast<const gfft::cx<float>*>(b->d_tw[c][kSetVariant[b->pset][c]]);
+  a.wide = b->pteam == 0 ? 1u : 0u;
+  a.nlevels = dp.nlevels;
+  a.nshares = dp.nshares;
+  a.level_base = dp.level_base;
+  a.slots = dp.domains * dp.per_dom;
+  a.wshare = dp.wshare;
+  a.lvl_off = dp.lvl_off;
+  a.own_off = dp.own_off;
+  a.own = dp.own;
   return a;
 }
 
@@ -587,10 +659,11 @@ int time_exec(gfft_batch_s* b, int mode, int reps, hipStream_t s, hipEvent_t e0,
   return GFFT_OK;
 }
 
-// Persistent scheduler: every (variant set, team factor) pair; a candidate
-// must win by 1 %; a candidate whose kernel does not launch is skipped.
+// Persistent scheduler: every (variant set, team factor) pair, team factor 0
+// being the global-level plan; a candidate must win by 1 %; a candidate whose
+// kernel does not launch is skipped.
 int tune_persistent(gfft_batch_s* b, int mode, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* best) {
-  static const int kTeamFactors[] = {1, 2, 4};
+  static const int kTeamFactors[] = {0, 1, 2, 4};
   int st = GFFT_OK, best_set = -1, best_factor = b->pteam;
   float best_ms = 1e30f;
   b->sched = GFFT_BATCH_PERSISTENT;
@@ -861,7 +934,7 @@ int gfft_batch_exec_noalloc(gfft_batch b, int mode, gfft_stream_t stream) {
   if (st != GFFT_OK) return st;
   const gfft_batch_s::DevPlan& dp = b->plan[mode];
   if (dp.nslots == 0) return GFFT_OK;
-  if (gfft::kCtrShares + (size_t)dp.claim_words > gfft::batch::kStaticCounterWords) return GFFT_ERR_SCRATCH_TOO_SMALL;
+  if (dp.ctr_words > gfft::batch::kStaticCounterWords) return GFFT_ERR_SCRATCH_TOO_SMALL;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   uint32_t* ctr = nullptr;
   if (gfft::batch::static_counters(&ctr, s) != hipSuccess) return GFFT_ERR_HIP;
@@ -980,7 +1053,7 @@ int gfft_batch_scheduler(gfft_batch b, int* scheduler) {
 }
 
 int gfft_batch_set_persist(gfft_batch b, int set, int team_factor) {
-  if (!b || set < 0 || set >= gfft::batch::kSets || team_factor < 1 || team_factor > 64) return GFFT_ERR_INVALID_ARG;
+  if (!b || set < 0 || set >= gfft::batch::kSets || team_factor < 0 || team_factor > 64) return GFFT_ERR_INVALID_ARG;
   drop_graphs(b);
   b->pset = set;
   b->pteam = team_factor;

@@ -36,6 +36,19 @@
 // failure bits into the sticky status word and zeroes the other counter
 // words, so the next launch (or graph replay) starts from zero without a
 // memset.
+//
+// With a global-level plan (team factor 0, WPlan in gfft_batch_sched.h) every
+// workgroup takes shares of every transform's level g in level g of the
+// launch: it claims its slot's list of the level (one compare-and-swap), runs
+// it, and publishes: its stores complete (in its XCD's L2), then it adds the
+// count it ran to its domain's done word.  Each domain's first workgroup (its
+// leader) sums the done words; at the level's share count it writes its XCD's
+// L2 back once (agent-scope release) if the domain did work, flags that, waits
+// for every working domain's flag and raises its domain's go flag, which the
+// domain's other workgroups watch.  Every wavefront then acquires at agent
+// scope (L1 and stale L2 lines dropped) before the next level.  A workgroup
+// that has waited steal_ticks claims and runs any list still unclaimed.
+// Waits are bounded and abortable as for the team plans.
 #include <map>
 #include <mutex>
 
@@ -239,6 +252,17 @@ __device__ __forceinline__ void stores_complete() { __asm__ volatile("s_waitcnt 
 // Later loads of this wavefront see what the L2 holds (the L1 is dropped).
 __device__ __forceinline__ void acquire_level() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
+// Agent-scope release followed by a flag store: the fence writes this XCD's
+// L2 back (buffer_wbl2 sc1); the explicit wait keeps the store behind the
+// write-back's completion (a release store compiled to the write-back and the
+// store with no wait between them, so the flag could be seen before the data
+// reached memory).
+__device__ __forceinline__ void release_store(uint32_t* p, uint32_t v) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // share m of level sg (tiles m, m + n, ... of n shares), then publish it as done
 template <int S>
 __device__ void run_share(const PersistArgs& a, const BXform& x, const PSeg& sg, uint32_t n, uint32_t m,
@@ -265,19 +289,25 @@ struct WaitState {
   uint64_t checked;  // wall clock of the last look at the failure and abort words (0: none yet)
   uint32_t next;     // where the next steal attempt starts (the scan runs downwards)
   uint32_t nap;      // sleep length of the next spin (grows to kMaxNap)
+  uint32_t phase;    // global-level plans: the leader's hand-over phase (wide_wait_step)
+  uint64_t steal_at; // global-level plans: wall clock of the next steal attempt
 };
 
 constexpr uint32_t kMaxNap = 16;            // x 512 clocks: about 4 us between looks at most
+constexpr uint32_t kMaxWideNap = 1;         // level hand-overs: looks at most ~0.2 us apart
 constexpr uint64_t kCheckTicks = 1000;      // failure / abort words read at most every 10 us
 
 // First wavefront only (all 64 lanes): one look at the level's k share words.
 // Returns kStepDone, kStepFail, kStepSpin, or the index (< k) of a share
 // this workgroup has just claimed (steal).  Every decision is made by lane 0
 // and broadcast, so the wavefront's control flow stays uniform.  The
-// launch-wide words (failure bits, the host's abort word in host memory) are
-// read at most every kCheckTicks, and the sleep between looks grows, so the
-// waiting workgroups put little traffic on the memory system.
-__device__ __forceinline__ uint32_t wait_step(const PersistArgs& a, const PSeg& sg, uint32_t k, WaitState* ws) {
+// launch-wide failure word is read at most every kCheckTicks, the host's
+// abort word (pinned host memory, read across PCIe) only by watch_host
+// workgroups (one per team) at that rate, which turn it into the failure
+// word; the sleep between looks grows, so the waiting workgroups put little
+// traffic on the memory system.
+__device__ __forceinline__ uint32_t wait_step(const PersistArgs& a, const PSeg& sg, uint32_t k, bool watch_host,
+                                              WaitState* ws) {
   const uint32_t lane = threadIdx.x;
   const uint32_t* w = a.ctr + kCtrShares + sg.claim0;
   bool mine_done = true, mine_free = false;
@@ -296,7 +326,7 @@ __device__ __forceinline__ uint32_t wait_step(const PersistArgs& a, const PSeg& 
       ws->checked = now;
       if (launch_failed(a)) {
         out = kStepFail;
-      } else if (a.host_abort &&
+      } else if (watch_host && a.host_abort &&
                  __hip_atomic_load(a.host_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
         __hip_atomic_fetch_or(a.ctr + kCtrFail, kFailAborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         out = kStepFail;
@@ -380,11 +410,11 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* sm
         __syncthreads();  // s_claimed is read before the next claim writes it
         if (mine) run_share<S>(a, x, sg, ns, own, smem);
       }
-      WaitState ws{0, 0, ns - 1 - member, 1};
+      WaitState ws{0, 0, ns - 1 - member, 1, 0, 0};
       if (tid < 64) ws.t0 = wall_clock64();  // checked = 0: the first look reads the launch-wide words
       for (;;) {
         if (tid < 64) {
-          const uint32_t st = wait_step(a, sg, ns, &ws);
+          const uint32_t st = wait_step(a, sg, ns, member == 0, &ws);
           if (tid == 0) *s_step = st;
         }
         __syncthreads();
@@ -435,11 +465,269 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* sm
   }
 }
 
+// ---------------------------------------------------------------------------
+// Global-level plans
+// ---------------------------------------------------------------------------
+// Sum of a value over the 64 lanes of the calling wavefront (every lane gets it).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Run the list of slot `slot` in level g (its shares, each a set of tiles of
+// one transform level); returns the number of shares.
+template <int S>
+__device__ __forceinline__ uint32_t run_list(const PersistArgs& a, uint32_t g, uint32_t slot, cx<float>* smem) {
+  const uint32_t b0 = a.own_off[g * a.slots + slot], b1 = a.own_off[g * a.slots + slot + 1];
+  for (uint32_t i = b0; i < b1; ++i) {
+    const WShare w = a.wshare[a.own[i]];
+    const PSeg sg = a.seg[w.seg];
+    const BXform x = a.xf[sg.xf];
+    run_set_tiles<S>(x, sg, w.first, w.step, w.count, a, smem);
+  }
+  return b1 - b0;
+}
+
+// Every thread's stores are in the L2; then thread 0 adds n to its domain's
+// done word (its domain's leader writes the L2 back once the level is
+// complete).  A workgroup without a domain releases itself (the L2 written
+// back before the add).
+__device__ __forceinline__ void publish(uint32_t* word, uint32_t n, bool own_release) {
+  stores_complete();
+  __syncthreads();
+  if (threadIdx.x == 0 && n) {
+    if (own_release)
+      __hip_atomic_fetch_add(word, n, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      __hip_atomic_fetch_add(word, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// First wavefront only: claim an unclaimed slot list of level g (a slot whose
+// workgroup has not run it), scanning `windows` windows of 64 claim words (one
+// load per lane each) from ws->next onward.  Returns the claimed slot, or
+// kStepSpin.
+__device__ __forceinline__ uint32_t wide_steal(const PersistArgs& a, uint32_t g, WaitState* ws, int windows) {
+  const uint32_t lane = threadIdx.x;
+  uint32_t* w = a.ctr + kCtrShares + (size_t)g * a.slots;
+  const uint32_t n = a.slots;
+  uint32_t out = kStepSpin;
+  for (int win = 0; win < windows && out == kStepSpin; ++win) {
+    const uint32_t base = ws->next;
+    const uint32_t s = (base + lane) % n;
+    const bool free_ = lane < n && ld_agent(w + s) == kShareFree &&
+                       a.own_off[g * n + s + 1] > a.own_off[g * n + s];  // lists with work only
+    uint64_t m = __ballot(free_);
+    while (m && out == kStepSpin) {
+      const uint32_t b = (uint32_t)(__ffsll((long long)m) - 1);
+      m &= m - 1;
+      uint32_t got = 0;
+      if (lane == 0) {
+        uint32_t expect = kShareFree;
+        got = __hip_atomic_compare_exchange_strong(w + (base + b) % n, &expect, kShareClaimed, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                  ? 1u
+                  : 0u;
+      }
+      if (__shfl(got, 0)) out = (base + b) % n;
+    }
+    ws->next = (base + 64) % n;
+  }
+  return out;
+}
+
+// First wavefront only: one look at level g's hand-over (block L of
+// kLevelWords words).  The domain's leader (or a workgroup without a domain)
+// runs the hand-over in phases kept in ws->phase: 0 = the done words sum to
+// the level's share count; 1 = its domain's L2 written back if the domain did
+// work, released[dom] set; 2 = every domain that did work has released, go[dom]
+// set.  The other workgroups watch go[dom].  Returns kStepDone, kStepFail,
+// kStepSpin or a slot whose list was just claimed (a steal, tried after a
+// wait of steal_ticks without completion).
+__device__ __forceinline__ uint32_t wide_wait_step(const PersistArgs& a, uint32_t g, uint32_t dom, bool leader,
+                                                   WaitState* ws) {
+  // only the leaders read the host's abort word (across PCIe; every waiting
+  // workgroup reading it at each level boundary held each look up for tens of
+  // microseconds); they turn it into the launch's failure word
+  const bool watch_host = leader;
+  const uint32_t lane = threadIdx.x;
+  uint32_t* L = a.ctr + a.level_base + (size_t)g * kLevelWords;
+  const uint32_t D = a.domains;
+  bool done = false;
+  if (leader) {
+    if (ws->phase == 0) {
+      uint32_t v = lane <= kMaxDomains ? ld_agent(L + kLevelDone + lane * kFlagStride) : 0u;
+      if (wave_sum(v) >= a.lvl_off[g + 1] - a.lvl_off[g]) ws->phase = 1;
+    }
+    if (ws->phase == 1) {
+      if (dom < D) {
+        const uint32_t mine = ld_agent(L + kLevelDone + dom * kFlagStride);
+        if (lane == 0 && mine)  // this domain's stores of the level: from its L2 to memory
+          release_store(L + kLevelReleased + dom * kFlagStride, 1u);
+      }
+      ws->phase = 2;
+    }
+    if (ws->phase == 2) {
+      bool ok = true;
+      if (lane < D)
+        ok = ld_agent(L + kLevelDone + lane * kFlagStride) == 0u ||
+             ld_agent(L + kLevelReleased + lane * kFlagStride) != 0u;
+      done = __all(ok);
+      if (done && lane == 0 && dom < D)
+        __hip_atomic_store(L + kLevelGo + dom * kFlagStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  } else {
+    done = ld_agent(L + kLevelGo + dom * kFlagStride) != 0u;
+  }
+  if (done) return kStepDone;
+  uint32_t out = kStepSpin;
+  bool steal = false;
+  if (lane == 0) {
+    const uint64_t now = wall_clock64();
+    const uint64_t el = now - ws->t0;
+    if (now - ws->checked > kCheckTicks || el > a.wait_ticks) {
+      ws->checked = now;
+      if (launch_failed(a)) {
+        out = kStepFail;
+      } else if (watch_host && a.host_abort &&
+                 __hip_atomic_load(a.host_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+        __hip_atomic_fetch_or(a.ctr + kCtrFail, kFailAborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out = kStepFail;
+      } else if (el > a.wait_ticks) {
+        __hip_atomic_fetch_or(a.ctr + kCtrFail, kFailTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out = kStepFail;
+      }
+    }
+    steal = out == kStepSpin && ws->phase == 0 && now >= ws->steal_at;
+    if (steal) ws->steal_at = now + a.steal_ticks;  // a scan that finds nothing waits again
+  }
+  out = __shfl(out, 0);
+  if (__shfl(steal ? 1u : 0u, 0)) out = wide_steal(a, g, ws, (int)((a.slots + 63) / 64));
+  if (out == kStepSpin) {
+    for (uint32_t i = 0; i < ws->nap; ++i) __builtin_amdgcn_s_sleep(8);
+    ws->nap = ws->nap < kMaxWideNap ? 2 * ws->nap : kMaxWideNap;
+  } else {
+    ws->nap = 1;
+  }
+  return out;
+}
+
+// The kernel's body for a global-level plan; sh as for persist_body.
+template <int S>
+__device__ __forceinline__ void wide_body(const PersistArgs& a, cx<float>* smem, uint32_t* sh) {
+  uint32_t *s_step = sh, *s_last = sh + 1, *s_claimed = sh + 2, *s_slot = sh + 3, *s_dom = sh + 4,
+           *s_poll = sh + 5;
+  const uint32_t tid = threadIdx.x;
+  uint32_t* claim_w = a.ctr + kCtrShares;  // per level and slot: its list claimed
+  if (tid == 0) {
+    const uint32_t d = xcc_id();
+    // no domain: no slot (only steals), releases its own work, leads itself
+    uint32_t slot = a.slots, dom = kMaxDomains, poll = 1;
+    if (d < a.domains) {
+      const uint32_t li = __hip_atomic_fetch_add(a.ctr + kCtrArrive + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      dom = d;
+      poll = li == 0 ? 1u : 0u;  // the domain's first workgroup leads its hand-overs
+      if (li < a.per_dom) slot = d * a.per_dom + li;
+    }
+    *s_slot = slot;
+    *s_dom = dom;
+    *s_poll = poll;
+  }
+  __syncthreads();
+  const uint32_t slot = *s_slot, dom = *s_dom;
+  const bool leader = *s_poll != 0u;
+  bool failed = false, solo_left = a.nsolo > 0;
+  for (uint32_t g = 0; g < a.nlevels && !failed; ++g) {
+    uint32_t* done = a.ctr + a.level_base + (size_t)g * kLevelWords + kLevelDone + dom * kFlagStride;
+    const bool own_release = dom >= a.domains;
+    uint32_t ran = 0;
+    if (slot < a.slots) {
+      if (tid == 0) {
+        uint32_t expect = kShareFree;
+        *s_claimed = __hip_atomic_compare_exchange_strong(claim_w + (size_t)g * a.slots + slot, &expect, kShareClaimed,
+                                                          __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         ? 1u
+                         : 0u;
+      }
+      __syncthreads();
+      const bool mine = *s_claimed != 0u;
+      __syncthreads();  // s_claimed is read before the next claim writes it
+      if (mine) ran = run_list<S>(a, g, slot, smem);
+    }
+    publish(done, ran, own_release);
+    WaitState ws{0, 0, (uint32_t)(((uint64_t)(slot < a.slots ? slot : 0) + 1) % a.slots), 1, 0, 0};
+    if (tid < 64) {
+      ws.t0 = wall_clock64();
+      ws.steal_at = ws.t0 + a.steal_ticks;
+    }
+    for (;;) {
+      if (tid < 64) {
+        const uint32_t st = wide_wait_step(a, g, dom, leader, &ws);
+        if (tid == 0) *s_step = st;
+      }
+      __syncthreads();
+      const uint32_t st = *s_step;
+      __syncthreads();  // s_step is read before the next step writes it
+      if (st == kStepDone) break;
+      if (st == kStepFail) {
+        failed = true;
+        break;
+      }
+      if (st < a.slots) {  // an absent workgroup's list
+        publish(done, run_list<S>(a, g, st, smem), own_release);
+      } else if (!leader && solo_left) {
+        // a solo transform while the level hands over (the leaders keep
+        // their hand-over duties)
+        if (tid == 0) *s_claimed = __hip_atomic_fetch_add(a.ctr + kCtrSolo, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t j = *s_claimed;
+        __syncthreads();
+        if (j < a.nsolo)
+          run_whole<S>(a, a.solo[j], smem);
+        else
+          solo_left = false;
+      }
+    }
+    if (!failed) acquire_level();  // the level's stores, from memory
+  }
+  // the solo transforms still queued
+  while (!failed && solo_left) {
+    if (tid == 0)
+      *s_claimed = launch_failed(a) ? kNoTeam
+                                    : __hip_atomic_fetch_add(a.ctr + kCtrSolo, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t j = *s_claimed;
+    __syncthreads();
+    if (j >= a.nsolo) break;
+    run_whole<S>(a, a.solo[j], smem);
+  }
+  // leave; the last workgroup out moves the failure bits and resets the counters
+  if (tid == 0) {
+    const uint32_t prev = __hip_atomic_fetch_add(a.ctr + kCtrExited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    *s_last = prev == gridDim.x - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (*s_last) {
+    if (tid == 0) {
+      const uint32_t f = ld_agent(a.ctr + kCtrFail);
+      if (f) __hip_atomic_fetch_or(a.status, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t words = a.level_base + a.nlevels * kLevelWords;
+    for (uint32_t i = tid; i < words; i += kBT)
+      if (a.ctr + i != a.status) __hip_atomic_store(a.ctr + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 template <int S>
 __global__ __launch_bounds__(kBT) void k_batch_persist(PersistArgs a) {
   extern __shared__ cx<float> smem[];
   __shared__ uint32_t sh[6];
-  persist_body<S>(a, smem, sh);
+  if (a.wide)
+    wide_body<S>(a, smem, sh);
+  else
+    persist_body<S>(a, smem, sh);
 }
 
 template <int S>

@@ -258,4 +258,207 @@ inline int check_plan(const PPlan& p) {
   return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Global-level plan (team factor 0).  A team confined to one domain moves its
+// transform through that domain's share of the memory system only (one XCD of
+// eight: a 192^3 FP32 round trip took 0.58 ms that way against 0.09 ms over
+// the whole device).  Here every workgroup takes part in every transform:
+// level g of the launch holds the g-th non-empty level of every transform,
+// each level's tiles are cut into shares of about equal cost (share j of a
+// transform level of m shares: tiles j, j + m, ..., so neighbouring shares
+// work on neighbouring tiles), and the host assigns the shares of a level to
+// the slots longest first onto the least-loaded slot.  A workgroup claims its
+// slot's list of the level with one compare-and-swap on the list's word, runs
+// it, and publishes the number of shares it ran (see the hand-over below); it
+// then waits for the level, and after a wait of steal_ticks claims and runs
+// any list still unclaimed (a slot whose workgroup has not arrived).  The
+// progress argument of the team plans holds unchanged: a list is claimed only
+// by a running workgroup, and any waiting workgroup may claim an unclaimed
+// one, so every level completes if one workgroup runs.
+// ---------------------------------------------------------------------------
+struct WShare {           // tiles first, first + step, ... (count of them) of one transform level
+  uint32_t seg;           // the transform level (index into WPlan::segs)
+  uint32_t first, step, count;
+};
+
+constexpr uint32_t kNoLevel = 0xffffffffu;
+
+struct WPlan {
+  std::vector<PSeg> segs;         // every transform's non-empty levels, in order per transform
+  std::vector<uint32_t> seg_level;  // global level of each seg (kNoLevel: a solo transform's)
+  std::vector<uint32_t> xseg0, xnseg;  // per transform: its segs
+  std::vector<uint32_t> solo;     // transforms run whole by one workgroup, longest first
+  uint32_t nlevels = 0;           // global levels of the launch
+  uint32_t slots = 1;             // workgroups the shares are assigned to
+  std::vector<WShare> shares;     // level-major
+  std::vector<uint32_t> lvl_off;  // nlevels + 1: level g's shares are [lvl_off[g], lvl_off[g + 1])
+  std::vector<uint32_t> own;      // share indices per (level, slot), in the order the slot runs them
+  std::vector<uint32_t> own_off;  // nlevels * slots + 1: (g, s) owns own[own_off[g*slots+s] .. own_off[g*slots+s+1])
+};
+
+// Counter words of a global-level launch: the common header, per level one
+// claim word per slot (its list of shares), then per level a block of
+// kLevelWords words, each group on cache lines of its own (kFlagStride words
+// apart):
+//   done[d], d <= kMaxDomains: shares completed by the workgroups of domain d
+//                (the last for workgroups without a domain);
+//   released[d]: domain d's L2 is written back for the level;
+//   go[d]:       domain d's workgroups may start the next level.
+// Level hand-over: a workgroup's stores complete (they are in its domain's
+// L2), then it adds its count to done[its domain] (no write-back).  The
+// domain's first workgroup (its leader) sums the done words; when they reach
+// the level's share count it writes its domain's L2 back once (an agent-scope
+// release) if its domain did any of the level's work and sets released[d],
+// waits until every domain that did work has done the same, and sets go[d],
+// which the domain's other workgroups watch.  Each XCD's L2 is written back
+// once per level, and no cache line is polled by more than one domain.
+constexpr uint32_t kFlagStride = 32;
+constexpr uint32_t kLevelDone = 0, kLevelReleased = (kMaxDomains + 1) * kFlagStride,
+                   kLevelGo = kLevelReleased + kMaxDomains * kFlagStride,
+                   kLevelWords = kLevelGo + kMaxDomains * kFlagStride;
+inline size_t wide_level_base(const WPlan& p) { return kCtrShares + (size_t)p.nlevels * p.slots; }
+inline size_t wide_counter_words(const WPlan& p) { return wide_level_base(p) + (size_t)p.nlevels * kLevelWords; }
+
+// tiles(t, l), cls(t, l) as for build_plan; cost(t, l): relative cost of one
+// tile of level l of transform t; every transform's k-th non-empty level goes
+// to global level k; a level is cut into about slots * shares_per_slot shares.
+// A transform whose work (sum of its levels' tile costs) is below
+// solo_factor slot shares of the total (share = total / slots) goes to the
+// solo queue instead: one workgroup runs it whole, taken from the queue while
+// workgroups wait at level hand-overs (and after the last level).
+template <typename TilesFn, typename ClsFn, typename CostFn>
+WPlan build_wide_plan(uint32_t ntransforms, int nlevels, const PLevelSpec* levels, TilesFn tiles, ClsFn cls,
+                      CostFn cost, uint32_t slots, uint32_t shares_per_slot = 2, double solo_factor = 0.0) {
+  WPlan p;
+  p.slots = std::max(slots, 1u);
+  std::vector<double> work(ntransforms, 0.0);
+  double all_work = 0;
+  for (uint32_t t = 0; t < ntransforms; ++t) {
+    for (int l = 0; l < nlevels; ++l) work[t] += (double)tiles(t, l) * std::max((double)cost(t, l), 1e-9);
+    all_work += work[t];
+  }
+  std::vector<uint8_t> is_solo(ntransforms, 0);
+  for (uint32_t t = 0; t < ntransforms; ++t)
+    is_solo[t] = work[t] > 0 && work[t] < solo_factor * all_work / p.slots ? 1 : 0;
+  std::vector<double> seg_cost;  // cost of one tile of each seg
+  p.xseg0.assign(ntransforms, 0);
+  p.xnseg.assign(ntransforms, 0);
+  for (uint32_t t = 0; t < ntransforms; ++t) {
+    uint32_t k = 0;
+    p.xseg0[t] = (uint32_t)p.segs.size();
+    for (int l = 0; l < nlevels; ++l) {
+      const uint32_t n = tiles(t, l);
+      if (!n) continue;
+      PSeg s;
+      s.xf = t;
+      s.axis = levels[l].axis;
+      s.inv = levels[l].inv;
+      s.cls = cls(t, l);
+      s.ntiles = n;
+      s.claim0 = 0;  // unused in global-level plans
+      p.segs.push_back(s);
+      p.seg_level.push_back(is_solo[t] ? kNoLevel : k);
+      seg_cost.push_back(std::max((double)cost(t, l), 1e-9));
+      if (!is_solo[t]) p.nlevels = std::max(p.nlevels, k + 1);
+      ++k;
+    }
+    p.xnseg[t] = (uint32_t)p.segs.size() - p.xseg0[t];
+  }
+  for (uint32_t t = 0; t < ntransforms; ++t)
+    if (is_solo[t]) p.solo.push_back(t);
+  std::stable_sort(p.solo.begin(), p.solo.end(), [&](uint32_t a, uint32_t b) { return work[a] > work[b]; });
+  p.lvl_off.push_back(0);
+  std::vector<std::vector<uint32_t>> per_slot(p.slots);
+  for (uint32_t g = 0; g < p.nlevels; ++g) {
+    double total = 0;
+    for (size_t i = 0; i < p.segs.size(); ++i)
+      if (p.seg_level[i] == g) total += seg_cost[i] * p.segs[i].ntiles;
+    const double target = total / ((double)p.slots * std::max(1u, shares_per_slot));
+    const uint32_t base = (uint32_t)p.shares.size();
+    std::vector<double> share_cost;
+    for (size_t i = 0; i < p.segs.size(); ++i) {
+      if (p.seg_level[i] != g) continue;
+      const uint32_t n = p.segs[i].ntiles;
+      const double c = seg_cost[i] * n;
+      uint32_t m = target > 0 ? (uint32_t)std::min<double>(n, std::max(1.0, std::floor(c / target + 0.5))) : 1u;
+      m = std::max(1u, std::min(m, n));
+      for (uint32_t j = 0; j < m; ++j) {
+        const uint32_t cnt = share_tiles(n, m, j);
+        p.shares.push_back(WShare{(uint32_t)i, j, m, cnt});
+        share_cost.push_back(seg_cost[i] * cnt);
+      }
+    }
+    // longest first onto the least-loaded slot (ties: the lowest slot)
+    std::vector<uint32_t> order(share_cost.size());
+    for (uint32_t j = 0; j < order.size(); ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return share_cost[a] > share_cost[b]; });
+    std::vector<double> load(p.slots, 0.0);
+    for (auto& v : per_slot) v.clear();
+    // slots are a few hundred: a linear scan for the least-loaded one
+    for (uint32_t j : order) {
+      uint32_t best = 0;
+      for (uint32_t s = 1; s < p.slots; ++s)
+        if (load[s] < load[best]) best = s;
+      load[best] += share_cost[j];
+      per_slot[best].push_back(base + j);
+    }
+    for (uint32_t s = 0; s < p.slots; ++s) {
+      p.own_off.push_back((uint32_t)p.own.size());
+      for (uint32_t j : per_slot[s]) p.own.push_back(j);
+    }
+    p.lvl_off.push_back((uint32_t)p.shares.size());
+  }
+  p.own_off.push_back((uint32_t)p.own.size());
+  return p;
+}
+
+// Invariants of a global-level plan: every tile of every seg in exactly one
+// share, shares of level g only over segs of level g, a transform's levels on
+// consecutive global levels in order, every share owned by exactly one slot of
+// its level.  Returns 0, or a positive code.
+inline int check_wide_plan(const WPlan& p) {
+  if (p.slots < 1 || p.lvl_off.size() != (size_t)p.nlevels + 1 || p.seg_level.size() != p.segs.size() ||
+      p.own_off.size() != (size_t)p.nlevels * p.slots + 1 || p.lvl_off.back() != p.shares.size())
+    return 1;
+  std::vector<uint8_t> solo(p.xseg0.size(), 0);
+  for (uint32_t t : p.solo) {
+    if (t >= solo.size() || solo[t]++) return 2;
+  }
+  for (size_t t = 0; t < p.xseg0.size(); ++t)
+    for (uint32_t i = p.xseg0[t]; i < p.xseg0[t] + p.xnseg[t]; ++i)
+      if (i >= p.segs.size() || p.segs[i].xf != t || (p.seg_level[i] == kNoLevel) != (solo[t] != 0)) return 2;
+  for (size_t i = 0; i < p.segs.size(); ++i) {
+    if (p.seg_level[i] == kNoLevel) continue;
+    if (p.seg_level[i] >= p.nlevels) return 2;
+    if (i > 0 && p.segs[i - 1].xf == p.segs[i].xf && p.seg_level[i] != p.seg_level[i - 1] + 1) return 2;
+    if ((i == 0 || p.segs[i - 1].xf != p.segs[i].xf) && p.seg_level[i] != 0) return 2;
+  }
+  std::vector<std::vector<uint8_t>> seen(p.segs.size());
+  for (size_t i = 0; i < p.segs.size(); ++i) seen[i].assign(p.segs[i].ntiles, 0);
+  for (uint32_t g = 0; g < p.nlevels; ++g)
+    for (uint32_t j = p.lvl_off[g]; j < p.lvl_off[g + 1]; ++j) {
+      const WShare& w = p.shares[j];
+      if (w.seg >= p.segs.size() || p.seg_level[w.seg] != g || w.step < 1) return 3;
+      if (w.count != share_tiles(p.segs[w.seg].ntiles, w.step, w.first)) return 4;
+      for (uint32_t k = 0; k < w.count; ++k) {
+        const uint32_t tile = w.first + k * w.step;
+        if (tile >= p.segs[w.seg].ntiles || seen[w.seg][tile]++) return 4;
+      }
+    }
+  for (size_t i = 0; i < p.segs.size(); ++i)
+    for (uint8_t v : seen[i])
+      if (v != (p.seg_level[i] == kNoLevel ? 0 : 1)) return 5;
+  std::vector<int> owned(p.shares.size(), 0);
+  for (uint32_t g = 0; g < p.nlevels; ++g)
+    for (uint32_t s = 0; s < p.slots; ++s)
+      for (uint32_t i = p.own_off[g * p.slots + s]; i < p.own_off[g * p.slots + s + 1]; ++i) {
+        const uint32_t j = p.own[i];
+        if (j < p.lvl_off[g] || j >= p.lvl_off[g + 1] || owned[j]++) return 6;
+      }
+  for (int v : owned)
+    if (v != 1) return 6;
+  return 0;
+}
+
 }  // namespace gfft

@@ -269,6 +269,15 @@ struct PersistArgs {
   uint64_t wait_ticks;         // bound of one wait, device wall-clock ticks
   uint64_t steal_ticks;        // a waiting member steals unclaimed shares after this long
   const cx<float>* tw[kClasses];  // twiddles of each class's variant in the set
+  // global-level plans (gfft_batch_sched.h WPlan; team, local, solo and
+  // team_ids unused): wide = 1
+  uint32_t wide;
+  uint32_t nlevels, nshares, slots;  // slots = domains * per_dom
+  uint32_t level_base;               // counter word of level 0's hand-over block (kLevelWords per level)
+  const WShare* wshare;              // level-major shares
+  const uint32_t* lvl_off;           // nlevels + 1
+  const uint32_t* own_off;           // nlevels * slots + 1
+  const uint32_t* own;               // share indices per (level, slot)
 };
 
 // Host side of the persistent kernels.

@@ -315,15 +315,20 @@ int gfft_batch_set_scheduler(gfft_batch batch, int scheduler);
 int gfft_batch_scheduler(gfft_batch batch, int* scheduler);
 
 /* Persistent scheduler: variant set (0..1) and team factor (1..64: a
- * transform gets a team when its work exceeds that many slot shares, a slot
- * share being the batch's work divided by the slots); query them and (if
+ * transform gets a team of workgroups in one XCD when its work exceeds that
+ * many slot shares, a slot share being the batch's work divided by the
+ * slots; 0: global-level plan, every workgroup takes shares of every
+ * transform's axis pass, one hand-over across all XCDs per pass, the
+ * smallest transforms run whole from a queue meanwhile); query them and (if
  * variants is not NULL) the set's variant per size class, 8 ints for the
- * sizes 32 ... 256.  set rebuilds the plans and synchronises the device. */
+ * sizes 32 ... 256.  set rebuilds the plans and synchronises the device.
+ * gfft_batch_tune tries team factors 0, 1, 2 and 4 with both sets. */
 int gfft_batch_set_persist(gfft_batch batch, int set, int team_factor);
 int gfft_batch_persist(gfft_batch batch, int* set, int* team_factor, int* variants);
 
 /* The persistent plan of `mode`: slots (workgroups of the launch), teams
- * (transforms run by two or more workgroups) and their members. */
+ * (transforms run by two or more workgroups) and their members (global-level
+ * plans: no teams, every slot a member). */
 int gfft_batch_plan_info(gfft_batch batch, int mode, uint32_t* slots, uint32_t* teams, uint32_t* members);
 
 /* ms per eager execution of the tuned mode measured by the last

@@ -18,6 +18,17 @@
 // fewer resident workgroups than slots, a team waits for a member that
 // cannot start; the bounded wait ends the simulation (failure reported)
 // instead of hanging.
+// The global-level plans (team factor 0, WPlan) get the same treatment: their
+// invariants on random batches, and a simulation of the kernel's wide body
+// (per level: claim the slot's list of shares with one compare-and-swap, run
+// it, add the count to the domain's done word; the domain's first workgroup
+// sums the done words, marks its domain's write-back, waits for every working
+// domain's and raises the domain's go flag, which the others watch; after a
+// delay, waiting workgroups claim and run lists of absent workgroups) under the
+// same residencies and spreads; checked: every tile once, every tile of level
+// g after every tile of level g - 1 has finished, counters zero after two
+// launches; planted mistake: no stealing with absent workgroups ends at the
+// bound.
 // Built and run by tests/test_batch_sched_host.py with g++ -pthread.
 #include <algorithm>
 #include <atomic>
@@ -279,10 +290,242 @@ static void no_stealing_with_absent_members_ends_by_the_bound() {
   CHECK(!t.failed && t.exactly_once && t.order_ok && t.domain_ok && t.reset);
 }
 
+
+static gfft::WPlan wide_of(const Batch& b, uint32_t slots, uint32_t q = 2, double solo_factor = 0.0) {
+  return gfft::build_wide_plan(
+      (uint32_t)b.tiles.size(), 6, kRoundTrip, [&](uint32_t t, int l) { return b.tiles[t][l]; },
+      [&](uint32_t t, int) { return b.cls[t]; }, [&](uint32_t t, int l) { return 1.0 + (t * 7 + l) % 3; }, slots, q,
+      solo_factor);
+}
+
+static void wide_plan_invariants_random() {
+  std::mt19937 rng(11);
+  for (int trial = 0; trial < 300; ++trial) {
+    const Batch b = random_batch(rng, 1 + (int)(rng() % 80));
+    const uint32_t slots = 1 + rng() % 300;
+    const double solo_factor = trial % 3 == 0 ? 0.0 : (trial % 3 == 1 ? 0.5 : 4.0);
+    const gfft::WPlan p = wide_of(b, slots, 1 + rng() % 4, solo_factor);
+    CHECK(gfft::check_wide_plan(p) == 0);
+    if (solo_factor == 0.0) CHECK(p.solo.empty());
+    size_t tiles = 0, planned = 0;
+    for (const auto& t : b.tiles)
+      for (uint32_t n : t) tiles += n;
+    for (const auto& sg : p.segs) planned += sg.ntiles;
+    CHECK(tiles == planned);
+    CHECK(gfft::wide_counter_words(p) ==
+          gfft::kCtrShares + (size_t)p.nlevels * p.slots + (size_t)p.nlevels * gfft::kLevelWords);
+  }
+  // one large transform among many small ones: its levels are cut over many slots
+  Batch b;
+  b.tiles = {{500, 500, 500, 500, 500, 500}};
+  b.cls = {6};
+  for (int i = 0; i < 50; ++i) {
+    b.tiles.push_back({8, 9, 9, 9, 9, 8});
+    b.cls.push_back(0);
+  }
+  const gfft::WPlan p = wide_of(b, 64);
+  CHECK(gfft::check_wide_plan(p) == 0 && p.nlevels == 6);
+  uint32_t big = 0;
+  for (uint32_t j = p.lvl_off[0]; j < p.lvl_off[1]; ++j) big += p.segs[p.shares[j].seg].xf == 0 ? 1u : 0u;
+  CHECK(big >= 32);
+}
+
+static SimResult simulate_wide(const gfft::WPlan& p, uint32_t domains, uint32_t per_dom, Ctr& ctr, int resident,
+                               bool steal, std::chrono::milliseconds wait_limit, unsigned seed, Spread spread) {
+  const size_t nseg = p.segs.size();
+  std::vector<std::vector<std::atomic<int>>> seen(nseg);
+  std::vector<std::atomic<uint32_t>> finished(p.nlevels);  // test-only: tiles run per global level
+  std::vector<uint32_t> level_tiles(p.nlevels, 0);
+  for (size_t i = 0; i < nseg; ++i) {
+    seen[i] = std::vector<std::atomic<int>>(p.segs[i].ntiles);
+    for (auto& v : seen[i]) v.store(0);
+    if (p.seg_level[i] != gfft::kNoLevel) level_tiles[p.seg_level[i]] += p.segs[i].ntiles;
+  }
+  for (auto& f : finished) f.store(0);
+  std::atomic<bool> order_bad{false};
+  Slots places(resident);
+  const int groups = (int)(domains * per_dom);
+  const size_t claim0 = gfft::kCtrShares, lvl0 = gfft::wide_level_base(p);
+  // test-only: which domains wrote their L2 back per level, and which ran shares
+  std::vector<std::vector<std::atomic<int>>> wb(p.nlevels), worked(p.nlevels);
+  for (uint32_t g = 0; g < p.nlevels; ++g) {
+    wb[g] = std::vector<std::atomic<int>>(domains);
+    worked[g] = std::vector<std::atomic<int>>(domains);
+    for (uint32_t k = 0; k < domains; ++k) {
+      wb[g][k].store(0);
+      worked[g][k].store(0);
+    }
+  }
+  std::atomic<bool> unreleased{false};
+  auto failed = [&] { return ctr[gfft::kCtrFail].load(std::memory_order_relaxed) != 0; };
+  // the slot's list of level g, by a workgroup of domain d; returns its share count
+  auto run_list = [&](uint32_t g, uint32_t s, uint32_t d, std::mt19937& rng) {
+    const uint32_t b0 = p.own_off[g * p.slots + s], b1 = p.own_off[g * p.slots + s + 1];
+    if (b1 > b0) {
+      if (g > 0 && finished[g - 1].load() != level_tiles[g - 1]) order_bad = true;
+      if (g > 0)  // the previous level's data must be in memory: every domain that wrote it wrote back
+        for (uint32_t k = 0; k < domains; ++k)
+          if (worked[g - 1][k].load() && !wb[g - 1][k].load()) unreleased = true;
+      worked[g][d].store(1);
+    }
+    for (uint32_t i = b0; i < b1; ++i) {
+      const gfft::WShare& w = p.shares[p.own[i]];
+      for (uint32_t k = 0; k < w.count; ++k) seen[w.seg][w.first + k * w.step].fetch_add(1);
+      if (rng() % 4 == 0) std::this_thread::sleep_for(std::chrono::microseconds(rng() % 40));
+      finished[g].fetch_add(w.count);
+    }
+    return b1 - b0;
+  };
+  auto claim = [&](uint32_t g, uint32_t s) {
+    uint32_t z = gfft::kShareFree;
+    return ctr[claim0 + (size_t)g * p.slots + s].compare_exchange_strong(z, gfft::kShareClaimed);
+  };
+  auto run_whole = [&](uint32_t t, std::mt19937& rng) {  // a solo transform, level after level, by one workgroup
+    for (uint32_t i = p.xseg0[t]; i < p.xseg0[t] + p.xnseg[t]; ++i) {
+      for (uint32_t k = 0; k < p.segs[i].ntiles; ++k) seen[i][k].fetch_add(1);
+      if (rng() % 4 == 0) std::this_thread::sleep_for(std::chrono::microseconds(rng() % 20));
+    }
+  };
+  auto wg = [&](int wgi) {
+    places.take();
+    std::mt19937 rng(seed * 131 + wgi);
+    const uint32_t d = spread == Spread::RoundRobin ? (uint32_t)wgi % domains
+                       : spread == Spread::AllInZero ? 0u
+                                                     : (uint32_t)(rng() % domains);
+    const uint32_t li = ctr[gfft::kCtrArrive + d].fetch_add(1);
+    const uint32_t slot = li < per_dom ? d * per_dom + li : p.slots;
+    const bool leader = li == 0;  // the domain's first workgroup leads its hand-overs
+    bool solo_left = !p.solo.empty();
+    auto take_solo = [&] {  // one ticket of the solo queue; false once it is empty
+      const uint32_t j = ctr[gfft::kCtrSolo].fetch_add(1);
+      if (j >= p.solo.size()) return false;
+      run_whole(p.solo[j], rng);
+      return true;
+    };
+    for (uint32_t g = 0; g < p.nlevels && !failed(); ++g) {
+      const size_t L = lvl0 + (size_t)g * gfft::kLevelWords;
+      std::atomic<uint32_t>& done = ctr[L + gfft::kLevelDone + d * gfft::kFlagStride];
+      uint32_t ran = 0;
+      if (slot < p.slots && claim(g, slot)) ran = run_list(g, slot, d, rng);
+      if (ran) done.fetch_add(ran, std::memory_order_relaxed);  // stores in the domain's L2
+      const uint32_t n = p.lvl_off[g + 1] - p.lvl_off[g];
+      int phase = 0;
+      auto level_done = [&] {
+        if (!leader) return ctr[L + gfft::kLevelGo + d * gfft::kFlagStride].load(std::memory_order_acquire) != 0;
+        if (phase == 0) {
+          uint32_t sum = 0;
+          for (uint32_t k = 0; k <= gfft::kMaxDomains; ++k) sum += ctr[L + gfft::kLevelDone + k * gfft::kFlagStride].load();
+          if (sum >= n) phase = 1;
+        }
+        if (phase == 1) {  // this domain's L2 written back, if it did work
+          if (ctr[L + gfft::kLevelDone + d * gfft::kFlagStride].load() != 0) {
+            wb[g][d].store(1);  // test-only: the write-back happened
+            ctr[L + gfft::kLevelReleased + d * gfft::kFlagStride].store(1, std::memory_order_release);
+          }
+          phase = 2;
+        }
+        for (uint32_t k = 0; k < domains; ++k)
+          if (ctr[L + gfft::kLevelDone + k * gfft::kFlagStride].load() != 0 &&
+              ctr[L + gfft::kLevelReleased + k * gfft::kFlagStride].load(std::memory_order_acquire) == 0)
+            return false;
+        ctr[L + gfft::kLevelGo + d * gfft::kFlagStride].store(1, std::memory_order_release);
+        return true;
+      };
+      const auto t0 = std::chrono::steady_clock::now();
+      auto steal_at = t0 + std::chrono::microseconds(50);
+      uint32_t next = slot < p.slots ? (slot + 1) % p.slots : 0;
+      while (!level_done()) {
+        if (failed()) break;
+        const auto now = std::chrono::steady_clock::now();
+        if (now - t0 > wait_limit) {
+          ctr[gfft::kCtrFail].fetch_or(gfft::kFailTimeout);
+          break;
+        }
+        if (steal && phase == 0 && now >= steal_at) {  // lists of absent workgroups
+          steal_at = now + std::chrono::microseconds(50);
+          for (uint32_t k = 0; k < p.slots; ++k) {
+            const uint32_t s = (next + k) % p.slots;
+            if (p.own_off[g * p.slots + s + 1] > p.own_off[g * p.slots + s] &&
+                ctr[claim0 + (size_t)g * p.slots + s].load() == gfft::kShareFree && claim(g, s)) {
+              done.fetch_add(run_list(g, s, d, rng), std::memory_order_relaxed);
+              next = (s + 1) % p.slots;
+              break;
+            }
+          }
+        } else if (!leader && solo_left) {  // a solo transform while the level hands over
+          solo_left = take_solo();
+        }
+        std::this_thread::yield();
+      }
+    }
+    while (!failed() && solo_left) solo_left = take_solo();
+    const uint32_t prev = ctr[gfft::kCtrExited].fetch_add(1, std::memory_order_acq_rel);
+    if (prev == (uint32_t)groups - 1) {  // last one out
+      ctr[gfft::kCtrStatus].fetch_or(ctr[gfft::kCtrFail].load());
+      for (size_t i = 0; i < ctr.size(); ++i)
+        if (i != gfft::kCtrStatus) ctr[i].store(0);
+    }
+    places.give();
+  };
+  std::vector<std::thread> th;
+  for (int w = 0; w < groups; ++w) th.emplace_back(wg, w);
+  for (auto& t : th) t.join();
+  SimResult r;
+  r.failed = ctr[gfft::kCtrStatus].exchange(0) != 0;
+  r.order_ok = !order_bad.load() && !unreleased.load();
+  for (size_t i = 0; i < nseg; ++i)
+    for (auto& v : seen[i]) r.exactly_once = r.exactly_once && v.load() == 1;
+  for (auto& c : ctr) r.reset = r.reset && c.load() == 0;
+  return r;
+}
+
+static void simulated_wide_kernel_completes() {
+  std::mt19937 rng(13);
+  const int resident_counts[] = {1, 3, 8, 1000};
+  const Spread spreads[] = {Spread::RoundRobin, Spread::AllInZero, Spread::Random};
+  for (int trial = 0; trial < 6; ++trial) {
+    const Batch b = random_batch(rng, 6 + (int)(rng() % 20));
+    const uint32_t domains = 4, per_dom = 6;
+    const gfft::WPlan p = wide_of(b, domains * per_dom, 1 + trial % 3, trial % 2 ? 2.0 : 0.0);
+    if (trial % 2) CHECK(!p.solo.empty());
+    CHECK(gfft::check_wide_plan(p) == 0);
+    for (Spread sp : spreads)
+      for (int r : resident_counts) {
+        Ctr ctr(gfft::wide_counter_words(p));
+        for (auto& c : ctr) c.store(0);
+        for (int rep = 0; rep < 2; ++rep) {
+          const SimResult s = simulate_wide(p, domains, per_dom, ctr, r, true, std::chrono::milliseconds(20000),
+                                            trial * 7 + r + rep, sp);
+          CHECK(!s.failed);
+          CHECK(s.order_ok);
+          CHECK(s.exactly_once);
+          CHECK(s.reset);
+        }
+      }
+  }
+}
+
+static void wide_no_stealing_with_absent_workgroups_ends_by_the_bound() {
+  Batch b;
+  b.tiles = {{64, 64, 64, 64, 64, 64}};
+  b.cls = {7};
+  const gfft::WPlan p = wide_of(b, 8);
+  Ctr ctr(gfft::wide_counter_words(p));
+  for (auto& c : ctr) c.store(0);
+  const SimResult s = simulate_wide(p, 1, 8, ctr, 1, false, std::chrono::milliseconds(200), 3, Spread::RoundRobin);
+  CHECK(s.failed);
+  CHECK(s.reset);
+  const SimResult t = simulate_wide(p, 1, 8, ctr, 1, true, std::chrono::milliseconds(20000), 4, Spread::RoundRobin);
+  CHECK(!t.failed && t.exactly_once && t.order_ok && t.reset);
+}
+
 int main() {
   plan_invariants_random();
   simulated_kernel_completes();
   no_stealing_with_absent_members_ends_by_the_bound();
+  wide_plan_invariants_random();
+  simulated_wide_kernel_completes();
+  wide_no_stealing_with_absent_workgroups_ends_by_the_bound();
   if (failures) {
     std::printf("%d failure(s)\n", failures);
     return 1;

@@ -212,7 +212,7 @@ def test_tuned_batch_matches_oracle():
         faster = "persistent" if chosen["tuned_ms"]["persistent"] <= chosen["tuned_ms"]["levels"] else "levels"
         assert chosen["scheduler"] == faster
         p = chosen["persist"]
-        assert 0 <= p["set"] <= 1 and p["team_factor"] in (1, 2, 4)
+        assert 0 <= p["set"] <= 1 and p["team_factor"] in (0, 1, 2, 4)
         assert set(chosen["variants"]) == {d[2] for d in dims}
         bufs, host = inputs(torch, dims, 600)
         b.set_data(bufs)
@@ -297,12 +297,12 @@ def test_bench_mix_256_seed0_matches_oracle():
     say(f"  worst R2C rel L2 {worst:.2e}")
 
 
-@pytest.mark.parametrize("set_index,chunk", [(0, 1), (0, 3), (1, 2), (1, 8)])
+@pytest.mark.parametrize("set_index,chunk", [(0, 1), (0, 3), (1, 2), (1, 8), (0, 0), (1, 0)])
 def test_persistent_sets_and_chunks_match_oracle(set_index, chunk):
     """Every variant set of the persistent kernel, with several team factors
-    (teams of many members down to all-solo plans), R2C against the FP64
-    oracle and the normalised round trip against the input (all sizes,
-    length-1 axes)."""
+    (teams of many members down to all-solo plans; 0 = the global-level plan,
+    every workgroup in every transform), R2C against the FP64 oracle and the
+    normalised round trip against the input (all sizes, length-1 axes)."""
     import torch
     dims = mix(30, seed=9) + [(256, 256, 256), (192, 192, 192), (1, 1, 48)]
     bufs, host = inputs(torch, dims, 800 + set_index)
@@ -312,7 +312,10 @@ def test_persistent_sets_and_chunks_match_oracle(set_index, chunk):
         assert b.persist()["set"] == set_index and b.persist()["team_factor"] == chunk
         info = b.plan_info(G.R2C)
         say(f"  plan {info}")
-        assert info["slots"] >= 1 and (info["teams"] == 0) == (info["members"] == 0)
+        if chunk == 0:
+            assert info["teams"] == 0 and info["members"] == info["slots"] >= 1
+        else:
+            assert info["slots"] >= 1 and (info["teams"] == 0) == (info["members"] == 0)
         b.set_data(bufs)
         b.exec(G.R2C)
         torch.cuda.synchronize()
@@ -436,6 +439,92 @@ def test_single_transform_teams_repeated_match_oracle():
                 b.set_data(bufs)
                 assert b.plan_info(G.ROUND_TRIP)["teams"] == 1
                 b.exec(G.ROUND_TRIP)
+                torch.cuda.synchronize()
+                assert b.status() == 0
+            err = O.rel_l2(bufs[0].cpu().numpy()[:, :, :n], host[0][:, :, :n])
+            assert err <= 1e-6, (n, rep, err)
+
+
+def test_global_level_plan_alone_mix_graph_abort():
+    """The global-level plan (team factor 0): every workgroup takes shares of
+    every transform's level and the level hand-over crosses XCDs (one agent-
+    scope release per workgroup and level, then an acquire).  Each result is
+    bitwise the same alone and in the mix, graph replays (with eager work
+    between them) equal the eager launch, an abort request and a zero wait
+    bound drain and report, and the next launch is right."""
+    import torch
+    dims = mix(40, seed=21) + [(192, 192, 192), (160, 160, 160)]
+    bufs, host = inputs(torch, dims, 1100)
+    scales = [1.0 / (nx * ny * nz) for nx, ny, nz in dims]
+    say(f"case global-level plan, {len(dims)} transforms")
+    with G.Batch(dims) as b:
+        b.set_persist(0, 0)
+        b.set_scales(c2r=scales)
+        b.set_data(bufs)
+        b.exec(G.ROUND_TRIP)
+        torch.cuda.synchronize()
+        assert b.status() == 0
+        eager = [x.clone() for x in bufs]
+        for i, ((nx, ny, nz), fp) in enumerate(zip(dims, host)):
+            err = O.rel_l2(bufs[i].cpu().numpy()[..., :nz], fp[..., :nz])
+            assert err <= 1e-6, ((nx, ny, nz), err)
+        for rep in range(3):
+            work = [torch.from_numpy(h).to("cuda:0") for h in host]
+            b.set_data(work)
+            b.exec_graph(G.ROUND_TRIP)
+            torch.cuda.synchronize()
+            _ = torch.ones(1000, device="cuda:0").sum().item()  # eager work between replays
+            assert b.status() == 0
+            for w, e in zip(work, eager):
+                assert torch.equal(w, e), rep
+        for i in (0, 5, len(dims) - 2, len(dims) - 1):
+            alone = [torch.from_numpy(host[i]).to("cuda:0")]
+            with G.Batch([dims[i]]) as one:
+                one.set_persist(0, 0)
+                one.set_scales(c2r=[scales[i]])
+                one.set_data(alone)
+                one.exec(G.ROUND_TRIP)
+                torch.cuda.synchronize()
+                assert one.status() == 0
+            assert torch.equal(alone[0], eager[i]), dims[i]
+        b.set_abort(True)
+        b.exec(G.ROUND_TRIP)
+        torch.cuda.synchronize()
+        assert b.status() == G.ERR_ABORTED
+        b.set_abort(False)
+        b.set_wait_limit(0.0)
+        b.exec_graph(G.ROUND_TRIP)
+        torch.cuda.synchronize()
+        assert b.status() in (0, G.ERR_TIMEOUT)
+        b.set_wait_limit(2.0)
+        work = [torch.from_numpy(h).to("cuda:0") for h in host]
+        b.set_data(work)
+        b.exec_graph(G.R2C)
+        torch.cuda.synchronize()
+        assert b.status() == 0
+        for (nx, ny, nz), buf, fp in zip(dims, work, host):
+            got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
+            assert O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), nz)) <= 1e-6
+
+
+def test_global_level_single_transforms_repeated_match_oracle():
+    """One transform alone under the global-level plan: its levels are cut
+    over every workgroup of every XCD, so each level's readers run on other
+    XCDs than its writers and must see the written data, not lines kept in
+    their L1 or L2.  Repeated round trips against the input (FP32 rel L2
+    <= 1e-6)."""
+    import torch
+    for n in (48, 64, 192):
+        dims = [(n, n, n)]
+        say(f"case global-level single transform {n}^3")
+        for rep in range(3):
+            bufs, host = inputs(torch, dims, 1300 + 10 * n + rep)
+            with G.Batch(dims) as b:
+                b.set_persist(rep % 2, 0)
+                b.set_scales(c2r=[1.0 / n ** 3])
+                b.set_data(bufs)
+                for _ in range(2):
+                    b.exec(G.ROUND_TRIP)
                 torch.cuda.synchronize()
                 assert b.status() == 0
             err = O.rel_l2(bufs[0].cpu().numpy()[:, :, :n], host[0][:, :, :n])
