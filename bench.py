@@ -271,6 +271,14 @@ def batch_bench(args):
         batch.set_scheduler(sch)
         sched_ms[name] = {"graph_ms": round(timed(lambda: batch.exec_graph(G.ROUND_TRIP, stream)), 5),
                           "launches": batch.tiles(G.ROUND_TRIP)[1]}
+    # the persistent kernel's global-level plan (team factor 0: every
+    # workgroup in every transform's passes), whatever the tuner kept
+    print("batch: persistent global-level plan, graph path", file=sys.stderr, flush=True)
+    batch.set_scheduler(G.Batch.PERSISTENT)
+    batch.set_persist(chosen["persist"]["set"], 0)
+    sched_ms["persistent_global_levels"] = {"graph_ms": round(timed(lambda: batch.exec_graph(G.ROUND_TRIP, stream)), 5),
+                                            "launches": batch.tiles(G.ROUND_TRIP)[1]}
+    batch.set_persist(chosen["persist"]["set"], chosen["persist"]["team_factor"])
     batch.set_scheduler(G.Batch.PERSISTENT if chosen["scheduler"] == "persistent" else G.Batch.LEVELS)
     status = max(status, batch.status())
     counts = {n: sum(1 for d in dims if d[0] == n) for n in BATCH_SIZES}
