@@ -287,6 +287,65 @@ def test_walk_order_does_not_change_results(shape, prec):
         assert torch.equal(a[..., :nz], b[..., :nz])
 
 
+@pytest.mark.parametrize("shape,prec", [((48, 64, 40), "f32"), ((30, 36, 50), "f64"), ((128, 128, 128), "f32")])
+def test_every_walk_mask_gives_the_same_bits(shape, prec):
+    """Per-pass walk directions (gfft_plan_set_walk_mask, all 64 masks on the
+    smallest shape, a spread of them otherwise) reorder tiles only: R2C and
+    normalised C2R outputs are bitwise those of mask 0."""
+    torch = _torch()
+    nx, ny, nz = shape
+    say(f"case walk masks {shape} {prec}")
+    fp = O.pad_real(O.random_field(nx, ny, nz, seed=78)).astype(np.float32 if prec == "f32" else np.float64)
+    masks = range(64) if nx * ny * nz < 150000 else (1, 2, 4, 8, 16, 32, 42, 49, 63)
+    with G.Plan(nx, ny, nz, precision=prec) as p:
+        if p.describe().get("zy_plane_kernel"):
+            p.set_fused(False)
+        p.set_scale(1.0, 1.0 / (nx * ny * nz))
+        p.set_walk_mask(0)
+        ref = upload(fp)
+        p.r2c(ref)
+        ref_spec = ref.clone()
+        p.c2r(ref)
+        for m in masks:
+            a = upload(fp)
+            p.set_walk_mask(m)
+            p.r2c(a)
+            torch.cuda.synchronize()
+            assert torch.equal(a, ref_spec), m
+            p.c2r(a)
+            torch.cuda.synchronize()
+            assert torch.equal(a[..., :nz], ref[..., :nz]), m
+
+
+@pytest.mark.parametrize("shape,prec", [((48, 64, 40), "f32"), ((30, 36, 50), "f64"), ((256, 128, 96), "f32")])
+def test_pass_orders_match_oracle(shape, prec):
+    """The four strided pass orders (gfft_plan_set_pass_order: R2C z,y,x or
+    z,x,y; C2R x,y,z or y,x,z) compute the same transform: each R2C against
+    the FP64 oracle and the default order (rel L2 within the precision's
+    bound), each normalised round trip against the input."""
+    torch = _torch()
+    nx, ny, nz = shape
+    say(f"case pass orders {shape} {prec}")
+    fp = O.pad_real(O.random_field(nx, ny, nz, seed=79)).astype(np.float32 if prec == "f32" else np.float64)
+    want = O.r2c_p1(fp.astype(np.float64), nz)
+    with G.Plan(nx, ny, nz, precision=prec) as p:
+        if p.describe().get("zy_plane_kernel"):
+            p.set_fused(False)
+        p.set_scale(1.0, 1.0 / (nx * ny * nz))
+        for order in range(4):
+            p.set_pass_order(order)
+            a = upload(fp)
+            p.r2c(a)
+            torch.cuda.synchronize()
+            got = G.complex_view(a, nx, ny, nz).cpu().numpy()
+            err = O.rel_l2(got, want)
+            assert err <= TOL[prec], (order, err)
+            p.c2r(a)
+            torch.cuda.synchronize()
+            err = O.rel_l2(a.cpu().numpy()[..., :nz], fp[..., :nz])
+            assert err <= TOL[prec], (order, err)
+
+
 def test_misaligned_data_pointers_rejected_before_any_launch():
     """Every kernel moves whole complex elements: a pointer off by one real
     is a layout error and is rejected (GFFT_ERR_INVALID_ARG) with the array
