@@ -1,9 +1,12 @@
-"""Host-only check of the persistent batched kernel's work plan
-(gpu_fft_amd/csrc/gfft_batch_sched.h): the native test builds random plans,
-checks their invariants, and simulates the kernel's arrival / claim / steal /
-wait / solo-queue / orphan / reset loop with threads under 1, 3, 8 and all
-resident workgroups, spread over the domains round robin, all in one domain
-or at random (see tests/native/test_batch_sched.cpp)."""
+"""Host-only check of the persistent batched kernel's work plans
+(gpu_fft_amd/csrc/gfft_batch_sched.h): the native test builds random team
+plans and global-level plans, checks their invariants, and simulates the
+kernel's loops with threads under 1, 3, 8 and all resident workgroups,
+spread over the domains round robin, all in one domain or at random: for team
+plans arrival / claim / steal / wait / solo-queue / orphan / reset; for
+global-level plans list claims, per-domain done words, the leaders' write-back
+and go flags, late steals of absent workgroups' lists, solo transforms during
+waits, reset (see tests/native/test_batch_sched.cpp)."""
 import os
 import subprocess
 
