@@ -4,7 +4,9 @@ per-GPU array instead of the batch's own allocation.  Checked against
 gfft_batch_exec under the persistent scheduler on the same inputs, on two
 calls in a row (the second with a smaller batch, so it runs on the counters
 the first call left behind), both bitwise (same kernel, same plan) and
-against the FP64 oracle (FP32 tolerance 1e-6); no failure bits afterwards."""
+against the FP64 oracle (FP32 tolerance 1e-6); no failure bits afterwards.
+Both plan kinds: team plans (the default team factor) and the global-level
+plan (team factor 0), whose counter words are laid out differently."""
 import sys
 
 import numpy as np
@@ -26,10 +28,13 @@ def _inputs(torch, dims, seed):
     return host, [torch.from_numpy(h).to("cuda:0") for h in host]
 
 
-def _run(torch, dims, seed, mode):
+def _run(torch, dims, seed, mode, team_factor):
     host, a = _inputs(torch, dims, seed)
     _, b = _inputs(torch, dims, seed)
     with G.Batch(dims) as ba, G.Batch(dims) as bb:
+        if team_factor is not None:
+            ba.set_persist(0, team_factor)
+            bb.set_persist(0, team_factor)
         ba.set_scheduler(G.Batch.PERSISTENT)
         ba.set_data(a)
         ba.exec(mode)
@@ -38,22 +43,25 @@ def _run(torch, dims, seed, mode):
         bb.exec_noalloc(mode)
         torch.cuda.synchronize()
         assert ba.status() == 0 and bb.status() == 0
-        teams = bb.plan_info(mode)["teams"]
-    return host, a, b, teams
+        info = bb.plan_info(mode)
+    return host, a, b, info
 
 
+@pytest.mark.parametrize("team_factor", [None, 0])
 @pytest.mark.parametrize("mode", [G.R2C, G.ROUND_TRIP])
-def test_noalloc_matches_exec_on_two_calls_second_smaller(mode):
+def test_noalloc_matches_exec_on_two_calls_second_smaller(mode, team_factor):
     import torch
     # first call: a large transform (a team of workgroups) among small ones;
     # second call: a smaller batch
     first = [(256, 256, 256), (192, 192, 192)] + [(32, 32, 32)] * 20 + [(64, 64, 64)] * 6 + [(1, 64, 64)]
     second = [(128, 128, 128)] + [(48, 48, 48)] * 5 + [(96, 1, 96)]
     for call, dims in enumerate((first, second)):
-        say(f"case noalloc mode={mode} call={call} transforms={len(dims)}")
-        host, a, b, teams = _run(torch, dims, 40 + 10 * call, mode)
-        if call == 0:
-            assert teams > 0  # the shared counter words are exercised
+        say(f"case noalloc mode={mode} team_factor={team_factor} call={call} transforms={len(dims)}")
+        host, a, b, info = _run(torch, dims, 40 + 10 * call, mode, team_factor)
+        if team_factor == 0:
+            assert info["teams"] == 0 and info["members"] == info["slots"]  # every workgroup in every level
+        elif call == 0:
+            assert info["teams"] > 0  # the shared counter words are exercised
         for (nx, ny, nz), x, y, fp in zip(dims, a, b, host):
             assert torch.equal(x, y), (nx, ny, nz)
             if mode == G.R2C:
