@@ -91,6 +91,10 @@ struct Seq {
   using COLS = ColsPass<float, INV, SC, P, W, true>;
   static constexpr int kTw = twiddle_count<SC>();
   static constexpr bool kPlane = F::CLS < kPersistPlaneClasses;  // z + y fused levels
+  // pipelined plane loop (the next plane in registers): classes 32-64; a 96
+  // or 128 plane holds 19 or 33 more entries per thread, which pushed their
+  // functions into accumulator registers and the kernel to one wave per SIMD
+  static constexpr bool kPlanePipe = kPlane && F::CLS <= 2;
   static constexpr int base_entries() {
     if constexpr (kPlane) {
       using PP = BPlane<F::CLS, F::VAR>;
@@ -149,6 +153,67 @@ struct Seq {
     }
   }
 
+  // x-planes first, first + step, ... (count of them) through the fused plane
+  // pass, software-pipelined like the rows and columns: the next plane's
+  // loads go out into registers before the current plane's LDS stages, whose
+  // barriers wait for LDS traffic only, and land in LDS once the current
+  // plane's stores have read it.  Planes of a level are disjoint, so loading
+  // the next before storing the current is safe in place.
+  __device__ static void planes(const BXform& x, bool inv, uint32_t first, uint32_t step, uint32_t count,
+                                const cx<float>* stw, cx<float>* smem) {
+    using PP = BPlaneL<F::CLS, F::VAR>;
+    static_assert(PP::NT == kBT, "one plane per 256-thread workgroup");
+    constexpr int U = (PP::PLANE + kBT - 1) / kBT;
+    constexpr bool kFull = U * kBT == PP::PLANE;
+    cx<float>* base = reinterpret_cast<cx<float>*>(x.data);
+    cx<float> nxt[U];
+    {
+      const cx<float>* g = base + (size_t)first * PP::PLANE;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = u * kBT + (int)threadIdx.x;
+        if (kFull || i < PP::PLANE) nxt[u] = ld_stream(g + i);
+      }
+    }
+    for (uint32_t k = 0; k < count; ++k) {
+      cx<float>* g = base + (size_t)(first + k * step) * PP::PLANE;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = u * kBT + (int)threadIdx.x;
+        if (kFull || i < PP::PLANE) smem[i] = nxt[u];
+      }
+      lds_barrier();
+      if (k + 1 < count) {
+        const cx<float>* gn = base + (size_t)(first + (k + 1) * step) * PP::PLANE;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int i = u * kBT + (int)threadIdx.x;
+          if (kFull || i < PP::PLANE) nxt[u] = ld_stream(gn + i);
+        }
+      }
+      if (!inv) {
+        PP::z_r2c(smem, stw);
+        lds_barrier();
+        // the forward scale belongs to this pass when x is length 1
+        const float sc = x.cls[2] < 0 ? x.scale_r2c : 1.0f;
+        PP::template y_pass<false>(smem, smem + PP::PLANE, stw, sc, sc != 1.0f);
+        lds_barrier();
+        for (int i = threadIdx.x; i < PP::PLANE; i += kBT) st_stream(g + i, smem[i]);
+      } else {
+        PP::template y_pass<true>(smem, smem + PP::PLANE, stw, 1.0f, false);
+        lds_barrier();
+        PP::z_c2r(smem, stw, x.scale_c2r, x.scale_c2r != 1.0f);
+        lds_barrier();
+        constexpr int H = PP::NZ / 2;
+        for (int i = threadIdx.x; i < PP::NY * H; i += kBT) {
+          const int y = i / H, m = i - y * H;
+          st_stream(g + y * PP::K + m, smem[y * PP::K + m]);
+        }
+      }
+      lds_barrier();  // the stores' LDS reads are done before the next plane lands
+    }
+  }
+
   template <bool INV>
   __device__ static void cols(const BXform& x, int axis, uint32_t first, uint32_t step, uint32_t count,
                               const cx<float>* stw, cx<float>* smem) {
@@ -189,7 +254,9 @@ struct Seq {
     __syncthreads();  // the table is complete (and the previous call's LDS reads are done)
     if (count > 0) {
       if (axis == kAxisPlane) {
-        if constexpr (kPlane) {
+        if constexpr (kPlanePipe) {
+          planes(x, inv != 0, first, step, count, stw, smem);
+        } else if constexpr (kPlane) {
           for (uint32_t k = 0; k < count; ++k)
             plane_tile<BPlane<F::CLS, F::VAR>>(x, inv != 0, first + k * step, stw, smem);
         }

@@ -125,6 +125,10 @@ struct PlaneShape<6> {
 template <int C, int V>
 using BPlane = PlanePass<float, typename BVar<C, V>::SC, PlaneShape<C>::PZ, typename BVar<C, V>::SC,
                          PlaneShape<C>::PY, PlaneShape<C>::CW>;
+// The same with LDS-only barriers in its y stage (pipelined plane loops).
+template <int C, int V>
+using BPlaneL = PlanePass<float, typename BVar<C, V>::SC, PlaneShape<C>::PZ, typename BVar<C, V>::SC,
+                          PlaneShape<C>::PY, PlaneShape<C>::CW, 1>;
 
 inline int class_of(int64_t n) {
   for (int c = 0; c < kClasses; ++c)

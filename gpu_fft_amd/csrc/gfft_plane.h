@@ -42,14 +42,16 @@ struct StrideLds {
 // tail of max(Ny, K) entries follows the plane.  Lanes past the last column
 // exchange there, and an odd Ny's last z row runs its exchange into it.  The
 // LDS a scratch would take goes to wider column groups (more threads).
-template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW>
+// BMY: the y stage's barriers (0: workgroup barriers; 1: LDS-only barriers,
+// for loops that keep the next plane's global loads in flight across them).
+template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW, int BMY = 0>
 struct PlanePass {
   using C = cx<T>;
   static constexpr int NZ = SCZ::N, NY = SCY::N, K = NZ / 2 + 1;
   using EZF = RegEngine<T, false, SCZ, PZ, 2>;
   using EZI = RegEngine<T, true, SCZ, PZ, 2>;
-  using EYF = RegEngine<T, false, SCY, PY, 0>;
-  using EYI = RegEngine<T, true, SCY, PY, 0>;
+  using EYF = RegEngine<T, false, SCY, PY, BMY>;
+  using EYI = RegEngine<T, true, SCY, PY, BMY>;
   static constexpr int TPTZ = NZ / PZ, TPTY = NY / PY;
   static constexpr int NT = CW * TPTY;
   static constexpr int GZ = NT / TPTZ;  // row pairs in flight
@@ -184,7 +186,7 @@ struct PlanePass {
   // y C2C of every kz column of the LDS plane, in place
   template <bool INV>
   __device__ __forceinline__ static void y_pass(C* plane, C* scratch, const C* __restrict__ tw, T sc, bool scaled) {
-    using E = RegEngine<T, INV, SCY, PY, 0>;
+    using E = RegEngine<T, INV, SCY, PY, BMY>;
     const int c = threadIdx.x % CW, t = threadIdx.x / CW;
     for (int k0 = 0; k0 < K; k0 += CW) {
       const int k = k0 + c;
@@ -199,7 +201,7 @@ struct PlanePass {
       }
       StrideLds<T> L{valid ? plane + k : scratch, valid ? K : 1};
       E::run(v, t, L, tw);
-      __syncthreads();  // every exchange read of the columns is done
+      stage_barrier<BMY>();  // every exchange read of the columns is done
       if (valid) {
 #pragma unroll
         for (int q = 0; q < E::NBL; ++q) {
