@@ -180,6 +180,8 @@ struct gfft_batch_s {
     gfft::PEntry* local = nullptr;
     uint32_t* solo = nullptr;
     uint32_t* team_ids = nullptr;
+    uint32_t* dom_off = nullptr;
+    uint32_t* dom_teams = nullptr;
     uint32_t nslots = 0;  // workgroups of the launch (domains * per_dom; 0: nothing to run)
     uint32_t domains = 1, per_dom = 1, nsolo = 0, claim_words = 0, tiles = 0, teams_k2 = 0, members = 0;
     // global-level plan (team factor 0)
@@ -265,6 +267,8 @@ void free_plans(gfft_batch_s* b) {
     if (p.local) (void)hipFree(p.local);
     if (p.solo) (void)hipFree(p.solo);
     if (p.team_ids) (void)hipFree(p.team_ids);
+    if (p.dom_off) (void)hipFree(p.dom_off);
+    if (p.dom_teams) (void)hipFree(p.dom_teams);
     if (p.wshare) (void)hipFree(p.wshare);
     if (p.lvl_off) (void)hipFree(p.lvl_off);
     if (p.own_off) (void)hipFree(p.own_off);
@@ -466,6 +470,8 @@ int build_plans(gfft_batch_s* b) {
     if (st == GFFT_OK) st = upload_vec(&dp.local, pl.local);
     if (st == GFFT_OK) st = upload_vec(&dp.solo, pl.solo);
     if (st == GFFT_OK) st = upload_vec(&dp.team_ids, pl.team_ids);
+    if (st == GFFT_OK) st = upload_vec(&dp.dom_off, pl.dom_off);
+    if (st == GFFT_OK) st = upload_vec(&dp.dom_teams, pl.dom_teams);
     if (st != GFFT_OK) {
       free_plans(b);
       return st;
@@ -509,6 +515,8 @@ gfft::batch::PersistArgs persist_args(const gfft_batch_s* b, int mode) {
   a.local = dp.local;
   a.solo = dp.solo;
   a.team_ids = dp.team_ids;
+  a.dom_off = dp.dom_off;
+  a.dom_teams = dp.dom_teams;
   a.domains = dp.domains;
   a.per_dom = dp.per_dom;
   a.nsolo = dp.nsolo;

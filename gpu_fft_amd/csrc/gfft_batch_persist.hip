@@ -31,7 +31,11 @@
 // being resident.  Every wait is bounded: it gives up when the launch's
 // wall-clock bound passes or when the host sets the batch's abort word
 // (pinned host memory, system-scope loads), and marks the launch failed; a
-// failed launch's workgroups stop computing and leave.  The last workgroup to
+// failed launch's workgroups stop computing and leave.  A workgroup that
+// finds the solo queue empty helps the teams of its own domain (same L2):
+// it claims free shares of each team's current level, runs them and marks
+// them done as a member would, after an agent-scope acquire (the level
+// before is complete in the L2), and leaves when those teams are complete.  The last workgroup to
 // leave runs the teams of which no member arrived (whole, alone), moves the
 // failure bits into the sticky status word and zeroes the other counter
 // words, so the next launch (or graph replay) starts from zero without a
@@ -91,12 +95,20 @@ struct Seq {
   using COLS = ColsPass<float, INV, SC, P, W, true>;
   static constexpr int kTw = twiddle_count<SC>();
   static constexpr bool kPlane = F::CLS < kPersistPlaneClasses;  // z + y fused levels
-  // pipelined plane loop (the next plane in registers): classes 32-64; a 96
-  // or 128 plane holds 19 or 33 more entries per thread, which pushed their
-  // functions into accumulator registers and the kernel to one wave per SIMD
-  static constexpr bool kPlanePipe = kPlane && F::CLS <= 2;
+  // plane loops: one wavefront per plane for 32 and 48 (a 256-thread plane
+  // of those leaves most threads idle in its stages), the workgroup on one
+  // plane, pipelined, for 64; 96 and 128 unpipelined (the next plane's 19 or
+  // 33 entries per thread pushed their functions into accumulator registers
+  // and the kernel to one wave per SIMD)
+  static constexpr bool kPlaneWave = kPlane && F::CLS <= 1;
+  static constexpr bool kPlanePipe = kPlane && F::CLS == 2;
+  static constexpr int kWaves = kBT / 64;
   static constexpr int base_entries() {
-    if constexpr (kPlane) {
+    if constexpr (kPlaneWave) {
+      using PP = BPlaneW<F::CLS, F::VAR>;
+      constexpr int e = kWaves * (PP::PLANE + PP::SCRATCH);
+      return F::lds_entries() > e ? F::lds_entries() : e;
+    } else if constexpr (kPlane) {
       using PP = BPlane<F::CLS, F::VAR>;
       return F::lds_entries() > PP::PLANE + PP::SCRATCH ? F::lds_entries() : PP::PLANE + PP::SCRATCH;
     } else {
@@ -214,6 +226,67 @@ struct Seq {
     }
   }
 
+  // x-planes through the plane pass, one wavefront per plane: wavefront w
+  // takes elements w, w + 4, ... of the tile list (first, first + step, ...)
+  // and runs each in an LDS region of its own with wavefront-level barriers
+  // only, the next plane's loads in flight meanwhile, as in planes().
+  __device__ static void planes_wave(const BXform& x, bool inv, uint32_t first, uint32_t step, uint32_t count,
+                                     const cx<float>* stw, cx<float>* smem) {
+    using PP = BPlaneW<F::CLS, F::VAR>;
+    constexpr int U = (PP::PLANE + 63) / 64;
+    constexpr bool kFull = U * 64 == PP::PLANE;
+    const uint32_t w = threadIdx.x >> 6;
+    const int lane = (int)(threadIdx.x & 63u);
+    cx<float>* L = smem + w * (PP::PLANE + PP::SCRATCH);
+    cx<float>* base = reinterpret_cast<cx<float>*>(x.data);
+    cx<float> nxt[U];
+    if (w < count) {
+      const cx<float>* g = base + (size_t)(first + w * step) * PP::PLANE;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = u * 64 + lane;
+        if (kFull || i < PP::PLANE) nxt[u] = ld_stream(g + i);
+      }
+    }
+    for (uint32_t k = w; k < count; k += kWaves) {
+      cx<float>* g = base + (size_t)(first + k * step) * PP::PLANE;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = u * 64 + lane;
+        if (kFull || i < PP::PLANE) L[i] = nxt[u];
+      }
+      wave_sync();
+      if (k + kWaves < count) {
+        const cx<float>* gn = base + (size_t)(first + (k + kWaves) * step) * PP::PLANE;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int i = u * 64 + lane;
+          if (kFull || i < PP::PLANE) nxt[u] = ld_stream(gn + i);
+        }
+      }
+      if (!inv) {
+        PP::z_r2c(L, stw);
+        wave_sync();
+        // the forward scale belongs to this pass when x is length 1
+        const float sc = x.cls[2] < 0 ? x.scale_r2c : 1.0f;
+        PP::template y_pass<false>(L, L + PP::PLANE, stw, sc, sc != 1.0f);
+        wave_sync();
+        for (int i = lane; i < PP::PLANE; i += 64) st_stream(g + i, L[i]);
+      } else {
+        PP::template y_pass<true>(L, L + PP::PLANE, stw, 1.0f, false);
+        wave_sync();
+        PP::z_c2r(L, stw, x.scale_c2r, x.scale_c2r != 1.0f);
+        wave_sync();
+        constexpr int H = PP::NZ / 2;
+        for (int i = lane; i < PP::NY * H; i += 64) {
+          const int y = i / H, m = i - y * H;
+          st_stream(g + y * PP::K + m, L[y * PP::K + m]);
+        }
+      }
+      wave_sync();  // the stores' LDS reads are done before the next plane lands
+    }
+  }
+
   template <bool INV>
   __device__ static void cols(const BXform& x, int axis, uint32_t first, uint32_t step, uint32_t count,
                               const cx<float>* stw, cx<float>* smem) {
@@ -254,7 +327,9 @@ struct Seq {
     __syncthreads();  // the table is complete (and the previous call's LDS reads are done)
     if (count > 0) {
       if (axis == kAxisPlane) {
-        if constexpr (kPlanePipe) {
+        if constexpr (kPlaneWave) {
+          planes_wave(x, inv != 0, first, step, count, stw, smem);
+        } else if constexpr (kPlanePipe) {
           planes(x, inv != 0, first, step, count, stw, smem);
         } else if constexpr (kPlane) {
           for (uint32_t k = 0; k < count; ++k)
@@ -332,7 +407,7 @@ __device__ __forceinline__ void release_store(uint32_t* p, uint32_t v) {
 
 // share m of level sg (tiles m, m + n, ... of n shares), then publish it as done
 template <int S>
-__device__ void run_share(const PersistArgs& a, const BXform& x, const PSeg& sg, uint32_t n, uint32_t m,
+__device__ __forceinline__ void run_share(const PersistArgs& a, const BXform& x, const PSeg& sg, uint32_t n, uint32_t m,
                           cx<float>* smem) {
   run_set_tiles<S>(x, sg, m, n, share_tiles(sg.ntiles, n, m), a, smem);
   stores_complete();  // this thread's stores are in the L2
@@ -348,7 +423,8 @@ __device__ __forceinline__ bool claim_share(const PersistArgs& a, const PSeg& sg
 }
 
 // Wait-step outcomes, decided by the first wavefront
-constexpr uint32_t kStepDone = 0xffffffffu, kStepFail = 0xfffffffeu, kStepSpin = 0xfffffffdu;
+constexpr uint32_t kStepDone = 0xffffffffu, kStepFail = 0xfffffffeu, kStepSpin = 0xfffffffdu,
+                   kStepRun = 0xfffffffcu;
 
 // State of one workgroup's wait (first wavefront).
 struct WaitState {
@@ -428,6 +504,133 @@ __device__ __forceinline__ uint32_t wait_step(const PersistArgs& a, const PSeg& 
   return out;
 }
 
+// First wavefront only (all 64 lanes): one look for work among the teams of
+// domain `dom` (the first 64 of its list) whose first member has arrived.
+// Lane i keeps team i's current level (the first whose shares are not all
+// done) in *lvl between looks.  Returns kStepDone (every such team is
+// complete), kStepFail, kStepSpin (nothing free now), or kStepRun with
+// *team, *seg and *share set: a share of a current level just claimed.
+// Timeouts and the failure word are handled as in wait_step; the host's
+// abort word is left to the teams' first members.
+__device__ __forceinline__ uint32_t help_step(const PersistArgs& a, uint32_t dom, uint32_t* lvl, WaitState* ws,
+                                              uint32_t* team, uint32_t* seg, uint32_t* share) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t t0 = a.dom_off[dom], nt = min(a.dom_off[dom + 1] - t0, 64u);
+  const uint32_t arrived = ld_agent(a.ctr + kCtrArrive + dom);
+  bool unfinished = false;
+  for (uint32_t i = 0; i < nt; ++i) {
+    const uint32_t t = a.dom_teams[t0 + i];
+    const PTeam tm = a.team[t];
+    if (arrived <= tm.slot0) continue;  // no member yet: possibly an orphan, run whole at the end
+    const uint32_t ns = tm.k * tm.q;
+    uint32_t cur = __shfl(*lvl, (int)i);
+    while (cur < tm.nseg) {
+      const PSeg sg = a.seg[tm.seg0 + cur];
+      uint32_t* w = a.ctr + kCtrShares + sg.claim0;
+      bool mine_done = true;
+      uint32_t free_j = kStepDone;
+      for (uint32_t j = lane; j < ns; j += 64) {
+        const uint32_t v = ld_agent(w + j);
+        mine_done = mine_done && v == kShareDone;
+        if (v == kShareFree && free_j == kStepDone) free_j = j;
+      }
+      if (__all(mine_done)) {
+        ++cur;
+        continue;
+      }
+      uint64_t m = __ballot(free_j != kStepDone);
+      while (m) {
+        const int b = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const uint32_t j = __shfl(free_j, b);
+        uint32_t got = 0;
+        if (lane == 0) {
+          uint32_t expect = kShareFree;
+          got = __hip_atomic_compare_exchange_strong(w + j, &expect, kShareClaimed, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                    ? 1u
+                    : 0u;
+        }
+        if (__shfl(got, 0)) {
+          if (lane == i) *lvl = cur;
+          *team = t;
+          *seg = tm.seg0 + cur;
+          *share = j;
+          ws->nap = 1;
+          ws->t0 = wall_clock64();
+          return kStepRun;
+        }
+      }
+      break;  // the level is under way with nothing free
+    }
+    if (lane == i) *lvl = cur;
+    unfinished = unfinished || cur < tm.nseg;
+  }
+  if (!unfinished) return kStepDone;
+  uint32_t out = kStepSpin;
+  if (lane == 0) {
+    const uint64_t now = wall_clock64();
+    const uint64_t el = now - ws->t0;
+    if (now - ws->checked > kCheckTicks || el > a.wait_ticks) {
+      ws->checked = now;
+      if (launch_failed(a)) {
+        out = kStepFail;
+      } else if (el > a.wait_ticks) {
+        __hip_atomic_fetch_or(a.ctr + kCtrFail, kFailTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out = kStepFail;
+      }
+    }
+  }
+  out = __shfl(out, 0);
+  if (out == kStepSpin) {
+    for (uint32_t i = 0; i < ws->nap; ++i) __builtin_amdgcn_s_sleep(8);
+    ws->nap = ws->nap < kMaxNap ? 2 * ws->nap : kMaxNap;
+  }
+  return out;
+}
+
+// A helper's loop; sh: the body's LDS words.  Returns true when the launch
+// failed.  The per-class tile functions save no callee-saved registers only
+// while no call of theirs may be a tail call (LLVM's no-CSR rule for local
+// functions); a call whose arguments all point outside the caller's frame
+// may be, so the helper passes local copies of the descriptors, as the other
+// callers do (with references into the plan arrays the 96 and 128 functions
+// saved registers into accumulator registers, and the kernel fell to one
+// wave per SIMD).
+template <int S>
+__device__ __forceinline__ bool help_teams(const PersistArgs& a, uint32_t dom, cx<float>* smem,
+                                                     uint32_t* sh) {
+  uint32_t *s_step = sh, *s_team = sh + 3, *s_seg = sh + 4, *s_share = sh + 5;
+  const uint32_t tid = threadIdx.x;
+  uint32_t lvl = 0;  // first wavefront: lane i, team i's current level
+  WaitState ws{0, 0, 0, 1, 0, 0};
+  if (tid < 64) ws.t0 = wall_clock64();
+  for (;;) {
+    if (tid < 64) {
+      uint32_t ht = 0, hs = 0, hj = 0;
+      const uint32_t st = help_step(a, dom, &lvl, &ws, &ht, &hs, &hj);
+      if (tid == 0) {
+        *s_step = st;
+        *s_team = ht;
+        *s_seg = hs;
+        *s_share = hj;
+      }
+    }
+    __syncthreads();
+    const uint32_t st = *s_step, ht = *s_team, hs = *s_seg, hj = *s_share;
+    __syncthreads();  // read before the next look writes them
+    if (st == kStepDone) return false;
+    if (st == kStepFail) return true;
+    if (st == kStepRun) {
+      acquire_level();  // the level before is complete in the L2
+      const PTeam tm = a.team[ht];
+      const BXform x = a.xf[ht];
+      const PSeg sg = a.seg[hs];
+      run_share<S>(a, x, sg, tm.k * tm.q, hj, smem);
+    }
+  }
+}
+
 // every level of transform t, by this workgroup alone
 template <int S>
 __device__ void run_whole(const PersistArgs& a, uint32_t t, cx<float>* smem) {
@@ -439,11 +642,12 @@ __device__ void run_whole(const PersistArgs& a, uint32_t t, cx<float>* smem) {
   }
 }
 
-// The kernel's body; sh: six LDS words (step, last, claimed, team, member, job).
+// The kernel's body; sh: seven LDS words (step, last, claimed, team, member,
+// job, domain).
 template <int S>
 __device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* smem, uint32_t* sh) {
   uint32_t *s_step = sh, *s_last = sh + 1, *s_claimed = sh + 2, *s_team = sh + 3, *s_member = sh + 4,
-           *s_job = sh + 5;
+           *s_job = sh + 5, *s_dom = sh + 6;
   const uint32_t tid = threadIdx.x;
   bool failed = false;
   // the next local slot of this workgroup's domain
@@ -460,9 +664,10 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* sm
     }
     *s_team = team;
     *s_member = member;
+    *s_dom = d;
   }
   __syncthreads();
-  const uint32_t team = *s_team, member = *s_member;
+  const uint32_t team = *s_team, member = *s_member, dom = *s_dom;
   if (team != kNoTeam) {  // a team member: per level the own share, then wait (stealing) for the level
     const PTeam tm = a.team[team];
     const BXform x = a.xf[team];
@@ -508,6 +713,8 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* sm
     if (j >= a.nsolo) break;
     run_whole<S>(a, a.solo[j], smem);
   }
+  // then help the teams of this workgroup's domain
+  if (!failed && dom < a.domains && a.dom_off[dom + 1] > a.dom_off[dom]) failed = help_teams<S>(a, dom, smem, sh);
   // leave; the last workgroup out runs the orphaned teams and resets the counters
   if (tid == 0) {
     const uint32_t prev = __hip_atomic_fetch_add(a.ctr + kCtrExited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -790,7 +997,7 @@ __device__ __forceinline__ void wide_body(const PersistArgs& a, cx<float>* smem,
 template <int S>
 __global__ __launch_bounds__(kBT) void k_batch_persist(PersistArgs a) {
   extern __shared__ cx<float> smem[];
-  __shared__ uint32_t sh[6];
+  __shared__ uint32_t sh[8];
   if (a.wide)
     wide_body<S>(a, smem, sh);
   else

@@ -29,6 +29,15 @@
 // the queue (one atomic ticket each) and runs each whole, level after level,
 // with nothing shared outside the workgroup.
 //
+// Helpers.  A workgroup that finds the solo queue empty helps the teams of
+// its own domain (same L2, so the L2-level hand-over holds for it as for a
+// member): for each team of the domain whose first member has arrived, it
+// finds the current level (the first whose shares are not all done: every
+// earlier level is complete), claims a free share of it (the same compare-
+// and-swap as a stealing member), runs it and marks it done; it leaves when
+// every such team is complete.  Teams of which no member has arrived are
+// left alone, so the orphan rule below still finds them untouched.
+//
 // Team synchronisation, per level: every share has a state word (0 =
 // unclaimed, 1 = claimed, 2 = done).  A member claims each of its shares in
 // turn (one compare-and-swap on the share's own word, so no two workgroups
@@ -46,9 +55,10 @@
 // walk through the levels.  A team of which no member arrives (its domain
 // received fewer workgroups than slot0 + 1, or none) is *orphaned*: the last
 // workgroup to leave finds it from the arrival counters and runs it whole.
-// Solo transforms wait for nobody.  Hence the launch completes, with every
-// tile run exactly once, whatever the residency and whatever the spread of
-// workgroups over the domains.
+// Solo transforms wait for nobody, and helpers wait only while a team they
+// may help is incomplete, which ends as above.  Hence the launch completes,
+// with every tile run exactly once, whatever the residency and whatever the
+// spread of workgroups over the domains.
 #pragma once
 #include <algorithm>
 #include <cstddef>
@@ -94,6 +104,9 @@ struct PPlan {
   std::vector<uint32_t> solo;   // solo transforms, longest first
   std::vector<uint32_t> team_ids;  // transforms with k >= 2
   uint32_t claim_words = 0;     // claim words of all team shares
+  // the teams of domain d, longest first: dom_teams[dom_off[d] .. dom_off[d + 1])
+  std::vector<uint32_t> dom_off;
+  std::vector<uint32_t> dom_teams;
 };
 
 struct PLevelSpec {
@@ -207,6 +220,13 @@ PPlan build_plan(uint32_t ntransforms, int nlevels, const PLevelSpec* levels, Ti
       p.solo.push_back(t);
     }
   }
+  // each domain's teams for its helpers, longest first (team_ids' order)
+  p.dom_off.assign(domains + 1, 0u);
+  for (uint32_t t : p.team_ids) p.dom_off[dom[t] + 1]++;
+  for (uint32_t d = 0; d < domains; ++d) p.dom_off[d + 1] += p.dom_off[d];
+  p.dom_teams.resize(p.team_ids.size());
+  std::vector<uint32_t> fill(p.dom_off.begin(), p.dom_off.end() - 1);
+  for (uint32_t t : p.team_ids) p.dom_teams[fill[dom[t]]++] = t;
   return p;
 }
 
@@ -214,7 +234,8 @@ PPlan build_plan(uint32_t ntransforms, int nlevels, const PLevelSpec* levels, Ti
 // slots of its domain, each local slot holds at most one member; every
 // transform with levels is a team or in the solo queue exactly once; claim
 // words of teams are disjoint; shares cover each level's tiles exactly
-// once.  Returns 0, or a positive code.
+// once; each team is listed once, under its own domain, for the helpers.
+// Returns 0, or a positive code.
 inline int check_plan(const PPlan& p) {
   if (p.domains < 1 || p.domains > kMaxDomains || p.local.size() != (size_t)p.domains * p.per_dom) return 1;
   std::vector<int> seen(p.teams.size(), 0);
@@ -253,6 +274,17 @@ inline int check_plan(const PPlan& p) {
           if (sg.claim0 + j >= p.claim_words) return 8;
           if (claim[sg.claim0 + j]++) return 9;
         }
+    }
+  }
+  if (p.dom_off.size() != (size_t)p.domains + 1 || p.dom_off[0] != 0 || p.dom_off.back() != p.dom_teams.size() ||
+      p.dom_teams.size() != p.team_ids.size())
+    return 10;
+  std::vector<int> listed(p.teams.size(), 0);
+  for (uint32_t d = 0; d < p.domains; ++d) {
+    if (p.dom_off[d] > p.dom_off[d + 1]) return 10;
+    for (uint32_t i = p.dom_off[d]; i < p.dom_off[d + 1]; ++i) {
+      const uint32_t t = p.dom_teams[i];
+      if (t >= p.teams.size() || p.teams[t].k < 2 || p.teams[t].dom != d || listed[t]++) return 11;
     }
   }
   return 0;

@@ -129,6 +129,10 @@ using BPlane = PlanePass<float, typename BVar<C, V>::SC, PlaneShape<C>::PZ, type
 template <int C, int V>
 using BPlaneL = PlanePass<float, typename BVar<C, V>::SC, PlaneShape<C>::PZ, typename BVar<C, V>::SC,
                           PlaneShape<C>::PY, PlaneShape<C>::CW, 1>;
+// One wavefront per plane (small planes): kz columns in groups of 64 / TPTY.
+template <int C, int V>
+using BPlaneW = PlanePass<float, typename BVar<C, V>::SC, PlaneShape<C>::PZ, typename BVar<C, V>::SC,
+                          PlaneShape<C>::PY, 64 / (BVar<C, V>::SC::N / PlaneShape<C>::PY), 2>;
 
 inline int class_of(int64_t n) {
   for (int c = 0; c < kClasses; ++c)
@@ -264,6 +268,8 @@ struct PersistArgs {
   const PEntry* local;         // domains * per_dom local slots, domain-major
   const uint32_t* solo;        // solo queue (transforms run whole by one workgroup)
   const uint32_t* team_ids;    // transforms run by teams
+  const uint32_t* dom_off;     // domains + 1: the teams of domain d are dom_teams[dom_off[d] ..
+  const uint32_t* dom_teams;   //   dom_off[d + 1]), longest first (the helpers' lists)
   uint32_t domains, per_dom;   // L2 domains (XCDs) and local slots per domain
   uint32_t nsolo, nteams;
   uint32_t claim_words;

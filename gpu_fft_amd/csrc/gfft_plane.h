@@ -43,7 +43,9 @@ struct StrideLds {
 // exchange there, and an odd Ny's last z row runs its exchange into it.  The
 // LDS a scratch would take goes to wider column groups (more threads).
 // BMY: the y stage's barriers (0: workgroup barriers; 1: LDS-only barriers,
-// for loops that keep the next plane's global loads in flight across them).
+// for loops that keep the next plane's global loads in flight across them;
+// 2: one wavefront per plane, NT = 64: lane ids, wavefront-level barriers,
+// so each wavefront of a workgroup can run a plane of its own).
 template <typename T, typename SCZ, int PZ, typename SCY, int PY, int CW, int BMY = 0>
 struct PlanePass {
   using C = cx<T>;
@@ -66,6 +68,10 @@ struct PlanePass {
   static_assert(SCRATCH >= K, "column scratch");
   static_assert(LDS_BYTES <= 160 * 1024, "plane does not fit in LDS");
   static_assert(NZ % 8 == 0, "FlatLds permutes inside blocks of 8 points");
+  static_assert(BMY != 2 || NT == 64, "one wavefront per plane");
+
+  // the thread's index in the plane's thread group
+  __device__ __forceinline__ static int lid() { return BMY == 2 ? (int)(threadIdx.x & 63u) : (int)threadIdx.x; }
 
   // all of a thread's loads are issued before its LDS stores (U in flight)
   __device__ __forceinline__ static void copy_in(C* plane, const C* src) {
@@ -74,12 +80,12 @@ struct PlanePass {
       C r[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int i = i0 + u * NT + (int)threadIdx.x;
+        const int i = i0 + u * NT + lid();
         if (i < PLANE) r[u] = ld_stream(src + i);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int i = i0 + u * NT + (int)threadIdx.x;
+        const int i = i0 + u * NT + lid();
         if (i < PLANE) plane[i] = r[u];
       }
     }
@@ -87,7 +93,7 @@ struct PlanePass {
 
   // z R2C of every row pair of the LDS plane, in place
   __device__ __forceinline__ static void z_r2c(C* plane, const C* __restrict__ tw) {
-    const int gi = threadIdx.x / TPTZ, t = threadIdx.x % TPTZ;
+    const int gi = lid() / TPTZ, t = lid() % TPTZ;
     constexpr int NI = (K + TPTZ - 1) / TPTZ;
     for (int pair = gi; pair < NPAIRS; pair += GZ) {
       const int ra = 2 * pair;
@@ -141,7 +147,7 @@ struct PlanePass {
   // z C2R of every row pair of the LDS plane, in place (reals, pitch 2K)
   __device__ __forceinline__ static void z_c2r(C* plane, const C* __restrict__ tw, T sc, bool scaled) {
     using RC = RowsC2R<T, SCZ, PZ, 1>;
-    const int gi = threadIdx.x / TPTZ, t = threadIdx.x % TPTZ;
+    const int gi = lid() / TPTZ, t = lid() % TPTZ;
     for (int pair = gi; pair < NPAIRS; pair += GZ) {
       const int ra = 2 * pair;
       const bool vb = ra + 1 < NY;
@@ -187,7 +193,7 @@ struct PlanePass {
   template <bool INV>
   __device__ __forceinline__ static void y_pass(C* plane, C* scratch, const C* __restrict__ tw, T sc, bool scaled) {
     using E = RegEngine<T, INV, SCY, PY, BMY>;
-    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+    const int c = lid() % CW, t = lid() / CW;
     for (int k0 = 0; k0 < K; k0 += CW) {
       const int k = k0 + c;
       const bool valid = k < K;

@@ -9,7 +9,9 @@
 // swap), runs it, marks it done, then waits until every share of the level
 // is done, stealing unclaimed shares after a short spin; then every workgroup
 // takes solo
-// transforms from the shared queue; the last workgroup to leave runs the
+// transforms from the shared queue, then helps the teams of its domain whose
+// first member has arrived (claims free shares of each one's current level);
+// the last workgroup to leave runs the
 // orphaned teams (no member arrived) and resets the counters.  Checked:
 // every tile runs exactly once and only after its transform's previous level
 // is complete, every team tile run by a member runs in the team's domain, no
@@ -17,7 +19,8 @@
 // row, for R down to 1.  Planted mistake: with stealing switched off and
 // fewer resident workgroups than slots, a team waits for a member that
 // cannot start; the bounded wait ends the simulation (failure reported)
-// instead of hanging.
+// instead of hanging.  Planted mistake: helpers that also help teams of which
+// no member has arrived run tiles the orphan rule runs again.
 // The global-level plans (team factor 0, WPlan) get the same treatment: their
 // invariants on random batches, and a simulation of the kernel's wide body
 // (per level: claim the slot's list of shares with one compare-and-swap, run
@@ -118,10 +121,13 @@ struct SimResult {
 
 using Ctr = std::vector<std::atomic<uint32_t>>;
 
-enum class Spread { RoundRobin, AllInZero, Random };
+// HalfOutside: the first half of the workgroups round robin over the domains,
+// the rest on an XCD the plan does not use (no slot, no domain to help)
+enum class Spread { RoundRobin, AllInZero, Random, HalfOutside };
 
 static SimResult simulate(const gfft::PPlan& p, Ctr& ctr, int resident, bool steal,
-                          std::chrono::milliseconds wait_limit, unsigned seed, Spread spread) {
+                          std::chrono::milliseconds wait_limit, unsigned seed, Spread spread,
+                          bool help_checks_arrival = true) {
   const size_t nseg = p.segs.size();
   std::vector<std::vector<std::atomic<int>>> seen(nseg);
   std::vector<std::atomic<uint32_t>> finished(nseg);  // test-only: tiles actually run per level
@@ -184,22 +190,70 @@ static SimResult simulate(const gfft::PPlan& p, Ctr& ctr, int resident, bool ste
       }
     }
   };
+  // a helper: free shares of the current level of each team of domain d
+  // whose first member has arrived, until every such team is complete
+  auto help = [&](uint32_t d, std::mt19937& rng) {
+    auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+      if (failed()) return;
+      bool unfinished = false, ran = false;
+      for (uint32_t i = p.dom_off[d]; i < p.dom_off[d + 1] && !ran; ++i) {
+        const uint32_t t = p.dom_teams[i];
+        const gfft::PTeam& tm = p.teams[t];
+        if (help_checks_arrival && ctr[gfft::kCtrArrive + d].load() <= tm.slot0) continue;
+        const uint32_t n = tm.k * tm.q;
+        uint32_t s = tm.seg0;
+        for (; s < tm.seg0 + tm.nseg; ++s) {
+          bool all = true;
+          for (uint32_t j = 0; j < n; ++j)
+            all = all && ctr[gfft::kCtrShares + p.segs[s].claim0 + j].load(std::memory_order_acquire) ==
+                             gfft::kShareDone;
+          if (!all) break;
+        }
+        if (s == tm.seg0 + tm.nseg) continue;  // complete
+        unfinished = true;
+        std::atomic<uint32_t>* share = &ctr[gfft::kCtrShares + p.segs[s].claim0];
+        for (uint32_t j = 0; j < n && !ran; ++j) {
+          uint32_t z = gfft::kShareFree;
+          if (share[j].compare_exchange_strong(z, gfft::kShareClaimed)) {
+            if (d != tm.dom) domain_bad = true;
+            run_tiles(s, n, j, rng);
+            share[j].store(gfft::kShareDone, std::memory_order_release);
+            ran = true;
+          }
+        }
+      }
+      if (!unfinished) return;
+      if (ran) {
+        t0 = std::chrono::steady_clock::now();
+      } else if (std::chrono::steady_clock::now() - t0 > wait_limit) {
+        ctr[gfft::kCtrFail].fetch_or(gfft::kFailTimeout);
+        return;
+      }
+      std::this_thread::yield();
+    }
+  };
   auto wg = [&](int w) {
     places.take();
     std::mt19937 rng(seed * 131 + w);
     const uint32_t d = spread == Spread::RoundRobin ? (uint32_t)w % p.domains
                        : spread == Spread::AllInZero ? 0u
-                                                     : (uint32_t)(rng() % p.domains);
-    const uint32_t li = ctr[gfft::kCtrArrive + d].fetch_add(1);
-    if (li < p.per_dom) {
-      const gfft::PEntry& e = p.local[(size_t)d * p.per_dom + li];
-      if (e.team != gfft::kNoTeam) run_member(e, d, rng);
+                       : spread == Spread::HalfOutside
+                           ? (w < groups / 2 ? (uint32_t)w % p.domains : p.domains)
+                           : (uint32_t)(rng() % p.domains);
+    if (d < p.domains) {  // a workgroup on an XCD outside the plan takes no slot
+      const uint32_t li = ctr[gfft::kCtrArrive + d].fetch_add(1);
+      if (li < p.per_dom) {
+        const gfft::PEntry& e = p.local[(size_t)d * p.per_dom + li];
+        if (e.team != gfft::kNoTeam) run_member(e, d, rng);
+      }
     }
     while (!failed()) {
       const uint32_t j = ctr[gfft::kCtrSolo].fetch_add(1);
       if (j >= p.solo.size()) break;
       run_whole(p.solo[j], rng);
     }
+    if (d < p.domains) help(d, rng);
     const uint32_t prev = ctr[gfft::kCtrExited].fetch_add(1, std::memory_order_acq_rel);
     if (prev == (uint32_t)groups - 1) {  // last one out
       for (uint32_t t : p.team_ids)
@@ -253,7 +307,7 @@ static void plan_invariants_random() {
 static void simulated_kernel_completes() {
   std::mt19937 rng(9);
   const int resident_counts[] = {1, 3, 8, 1000};
-  const Spread spreads[] = {Spread::RoundRobin, Spread::AllInZero, Spread::Random};
+  const Spread spreads[] = {Spread::RoundRobin, Spread::AllInZero, Spread::Random, Spread::HalfOutside};
   for (int trial = 0; trial < 6; ++trial) {
     const Batch b = random_batch(rng, 6 + (int)(rng() % 20));
     const gfft::PPlan p = plan_of(b, 4, 6, 1 + trial % 3);
@@ -519,10 +573,38 @@ static void wide_no_stealing_with_absent_workgroups_ends_by_the_bound() {
   CHECK(!t.failed && t.exactly_once && t.order_ok && t.reset);
 }
 
+// Two teams of four in one domain of eight slots; only the first four
+// workgroups land there (the rest on an XCD outside the plan), so the second
+// team is orphaned while the first team's members, done, help.  Helpers that
+// skip the arrival check run the orphan's tiles, and the last workgroup out
+// runs them again.
+static void helpers_leave_orphaned_teams_alone() {
+  Batch b;
+  b.tiles = {{40, 40, 40, 40, 40, 40}, {40, 40, 40, 40, 40, 40}};
+  b.cls = {5, 5};
+  for (int i = 0; i < 4; ++i) {
+    b.tiles.push_back({1, 1, 1, 1, 1, 1});
+    b.cls.push_back(0);
+  }
+  const gfft::PPlan p = plan_of(b, 1, 8);
+  CHECK(gfft::check_plan(p) == 0);
+  CHECK(p.team_ids.size() == 2 && p.teams[0].k == 4 && p.teams[1].k == 4);
+  CHECK(p.teams[0].slot0 + p.teams[1].slot0 == 4);  // one team takes slots 0-3, the other 4-7
+  Ctr ctr(gfft::counter_words(p));
+  for (auto& c : ctr) c.store(0);
+  const SimResult good =
+      simulate(p, ctr, 1000, true, std::chrono::milliseconds(20000), 5, Spread::HalfOutside, true);
+  CHECK(!good.failed && good.exactly_once && good.order_ok && good.domain_ok && good.reset);
+  const SimResult bad =
+      simulate(p, ctr, 1000, true, std::chrono::milliseconds(20000), 6, Spread::HalfOutside, false);
+  CHECK(!bad.exactly_once);
+}
+
 int main() {
   plan_invariants_random();
   simulated_kernel_completes();
   no_stealing_with_absent_members_ends_by_the_bound();
+  helpers_leave_orphaned_teams_alone();
   wide_plan_invariants_random();
   simulated_wide_kernel_completes();
   wide_no_stealing_with_absent_workgroups_ends_by_the_bound();
