@@ -363,6 +363,7 @@ def _bind_batch(L):
     for name, args in (("gfft_batch_set_scheduler", [vp, ctypes.c_int]), ("gfft_batch_scheduler", [vp, ip]),
                        ("gfft_batch_set_persist", [vp, ctypes.c_int, ctypes.c_int]),
                        ("gfft_batch_persist", [vp, ip, ip, ip]),
+                       ("gfft_batch_set_span", [vp, ctypes.c_int]), ("gfft_batch_span", [vp, ip]),
                        ("gfft_batch_set_wait_limit", [vp, ctypes.c_double]),
                        ("gfft_batch_set_abort", [vp, ctypes.c_int]), ("gfft_batch_status", [vp, ip]),
                        ("gfft_batch_plan_info", [vp, ctypes.c_int] + [ctypes.POINTER(ctypes.c_uint32)] * 3),
@@ -456,6 +457,17 @@ class Batch:
         shares; 0 = the global-level plan, every workgroup in every
         transform's axis passes)."""
         _check(lib().gfft_batch_set_persist(self.handle, int(set_index), int(team_factor)), "gfft_batch_set_persist")
+
+    def set_span(self, parts: int) -> None:
+        """Team plans: the most XCDs one team may span (1..8, default 2);
+        a transform whose share of workgroups exceeds one XCD's slots gets a
+        team over that many XCDs."""
+        _check(lib().gfft_batch_set_span(self.handle, int(parts)), "gfft_batch_set_span")
+
+    def span(self) -> int:
+        v = ctypes.c_int()
+        _check(lib().gfft_batch_span(self.handle, ctypes.byref(v)), "gfft_batch_span")
+        return v.value
 
     def persist(self) -> dict:
         s, c = ctypes.c_int(), ctypes.c_int()

@@ -174,9 +174,11 @@ struct gfft_batch_s {
   int sched = GFFT_BATCH_PERSISTENT;
   int pset = 0, pteam = 2;            // variant set, team factor (a team above pteam slot shares of
                                       // work; 0: global-level plans, every workgroup in every transform)
+  int pspan = 2;                      // team plans: XCDs a team may span
   struct DevPlan {
     gfft::PSeg* seg = nullptr;
     gfft::PTeam* team = nullptr;
+    gfft::PPart* parts = nullptr;
     gfft::PEntry* local = nullptr;
     uint32_t* solo = nullptr;
     uint32_t* team_ids = nullptr;
@@ -264,6 +266,7 @@ void free_plans(gfft_batch_s* b) {
   for (auto& p : b->plan) {
     if (p.seg) (void)hipFree(p.seg);
     if (p.team) (void)hipFree(p.team);
+    if (p.parts) (void)hipFree(p.parts);
     if (p.local) (void)hipFree(p.local);
     if (p.solo) (void)hipFree(p.solo);
     if (p.team_ids) (void)hipFree(p.team_ids);
@@ -429,7 +432,7 @@ int build_plans(gfft_batch_s* b) {
           domains * per_dom, kWideSharesPerSlot, kWideSoloFactor);
       if (check_wide_plan(wp) != 0) return GFFT_ERR_INVALID_ARG;  // cannot happen: build_wide_plan's construction
       std::vector<PTeam> xt(b->count);  // each transform's segs, for the solo transforms' whole runs
-      for (int t = 0; t < b->count; ++t) xt[t] = PTeam{wp.xseg0[t], wp.xnseg[t], 1u, 1u, 0u, 0u};
+      for (int t = 0; t < b->count; ++t) xt[t] = PTeam{wp.xseg0[t], wp.xnseg[t], 1u, 1u, 0u, 0u, 0u, 0u};
       int st = upload_vec(&dp.seg, wp.segs);
       if (st == GFFT_OK) st = upload_vec(&dp.team, xt);
       if (st == GFFT_OK) st = upload_vec(&dp.solo, wp.solo);
@@ -463,10 +466,11 @@ int build_plans(gfft_batch_s* b) {
           for (int l = 0; l < kModeLevels[m]; ++l) levels += tiles(t, l) ? 1.0 : 0.0;
           return (double)x.nrows * x.K * levels;
         },
-        domains, per_dom, (double)b->pteam, 2, kSharesPerMember);
+        domains, per_dom, (double)b->pteam, 2, kSharesPerMember, (uint32_t)b->pspan);
     if (check_plan(pl) != 0) return GFFT_ERR_INVALID_ARG;  // cannot happen: build_plan's construction
     int st = upload_vec(&dp.seg, pl.segs);
     if (st == GFFT_OK) st = upload_vec(&dp.team, pl.teams);
+    if (st == GFFT_OK) st = upload_vec(&dp.parts, pl.parts);
     if (st == GFFT_OK) st = upload_vec(&dp.local, pl.local);
     if (st == GFFT_OK) st = upload_vec(&dp.solo, pl.solo);
     if (st == GFFT_OK) st = upload_vec(&dp.team_ids, pl.team_ids);
@@ -512,6 +516,7 @@ gfft::batch::PersistArgs persist_args(const gfft_batch_s* b, int mode) {
   a.xf = b->d_xf;
   a.seg = dp.seg;
   a.team = dp.team;
+  a.parts = dp.parts;
   a.local = dp.local;
   a.solo = dp.solo;
   a.team_ids = dp.team_ids;
@@ -667,18 +672,20 @@ int time_exec(gfft_batch_s* b, int mode, int reps, hipStream_t s, hipEvent_t e0,
   return GFFT_OK;
 }
 
-// Persistent scheduler: every (variant set, team factor) pair, team factor 0
-// being the global-level plan; a candidate must win by 1 %; a candidate whose
-// kernel does not launch is skipped.
+// Persistent scheduler: every variant set with each (team factor, span)
+// pair, team factor 0 being the global-level plan; a candidate must win by
+// 1 %; a candidate whose kernel does not launch is skipped.
 int tune_persistent(gfft_batch_s* b, int mode, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* best) {
-  static const int kTeamFactors[] = {0, 1, 2, 4};
-  int st = GFFT_OK, best_set = -1, best_factor = b->pteam;
+  static const int kCandidates[][2] = {{0, 2}, {1, 2}, {2, 2}, {4, 2}, {1, 1}, {2, 1}};
+  int st = GFFT_OK, best_set = -1, best_factor = b->pteam, best_span = b->pspan;
   float best_ms = 1e30f;
   b->sched = GFFT_BATCH_PERSISTENT;
   for (int set = 0; set < gfft::batch::kSets && st == GFFT_OK; ++set)
-    for (int f : kTeamFactors) {
+    for (const auto& c : kCandidates) {
+      const int f = c[0];
       b->pset = set;
       b->pteam = f;
+      b->pspan = c[1];
       if ((st = build_plans(b)) != GFFT_OK) break;
       float ms = 0;
       const int rt = time_exec(b, mode, reps, s, e0, e1, &ms);
@@ -692,12 +699,14 @@ int tune_persistent(gfft_batch_s* b, int mode, int reps, hipStream_t s, hipEvent
         best_ms = ms;
         best_set = set;
         best_factor = f;
+        best_span = c[1];
       }
     }
   if (st != GFFT_OK) return st;
   if (best_set < 0) return GFFT_ERR_HIP;
   b->pset = best_set;
   b->pteam = best_factor;
+  b->pspan = best_span;
   *best = best_ms;
   return build_plans(b);
 }
@@ -1066,6 +1075,19 @@ int gfft_batch_set_persist(gfft_batch b, int set, int team_factor) {
   b->pset = set;
   b->pteam = team_factor;
   return build_plans(b);
+}
+
+int gfft_batch_set_span(gfft_batch b, int parts) {
+  if (!b || parts < 1 || parts > (int)gfft::kMaxDomains) return GFFT_ERR_INVALID_ARG;
+  drop_graphs(b);
+  b->pspan = parts;
+  return build_plans(b);
+}
+
+int gfft_batch_span(gfft_batch b, int* parts) {
+  if (!b || !parts) return GFFT_ERR_INVALID_ARG;
+  *parts = b->pspan;
+  return GFFT_OK;
 }
 
 int gfft_batch_persist(gfft_batch b, int* set, int* team_factor, int* variants) {

@@ -9,10 +9,11 @@
 // A solo transform needs no synchronisation beyond the workgroup barrier
 // between levels (one workgroup's stores are visible to its own later loads).
 //
-// Every workgroup that touches a team's array runs in the team's domain, so
-// they all go through one L2, and a level's hand-over needs L2-level
-// ordering only (the ordering the memory model uses for a workgroup whose
-// waves are split over CUs of one XCD):
+// Every workgroup that touches a team's array runs in the team's domain (or,
+// for a span team, in one of its parts' domains), so within a part they all
+// go through one L2, and a level's hand-over needs L2-level ordering only
+// (the ordering the memory model uses for a workgroup whose waves are split
+// over CUs of one XCD):
 //   run a share: claim it (compare-and-swap on the share's word), run its
 //             tiles, every thread waits for its stores to complete
 //             (s_waitcnt vmcnt(0): they are in the L2), workgroup barrier,
@@ -22,7 +23,12 @@
 //             wavefront runs an agent-scope acquire fence (buffer_inv sc1:
 //             drops the CU's L1 lines, keeps the L2's dirty lines) before
 //             it loads the next level.
-// No L2 write-back happens inside the launch.  (A workgroup-scope L1
+// A span team's level also waits, once every share is done, until each part
+// that ran shares (recorded in the done words: kShareDone + part) has been
+// released: one workgroup of the part (the first to claim the part's release
+// word) writes its XCD's L2 back (agent-scope release fence, then the flag),
+// and the acquire after the wait drops stale lines of the other parts' L2s.
+// Otherwise no L2 write-back happens inside the launch.  (A workgroup-scope L1
 // invalidate, buffer_inv sc0, is not enough outside threadgroup-split mode:
 // with it, lines a member had read in an earlier level were now and then
 // served stale from its L1.)  While
@@ -405,15 +411,17 @@ __device__ __forceinline__ void release_store(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// share m of level sg (tiles m, m + n, ... of n shares), then publish it as done
+// share m of level sg (tiles m, m + n, ... of n shares), then publish it as
+// done by a workgroup of team part `part`
 template <int S>
-__device__ __forceinline__ void run_share(const PersistArgs& a, const BXform& x, const PSeg& sg, uint32_t n, uint32_t m,
-                          cx<float>* smem) {
+__device__ __forceinline__ void run_share(const PersistArgs& a, const BXform& x, const PSeg& sg, uint32_t n,
+                                          uint32_t m, uint32_t part, cx<float>* smem) {
   run_set_tiles<S>(x, sg, m, n, share_tiles(sg.ntiles, n, m), a, smem);
   stores_complete();  // this thread's stores are in the L2
   __syncthreads();    // every thread's
   if (threadIdx.x == 0)
-    __hip_atomic_store(a.ctr + kCtrShares + sg.claim0 + m, kShareDone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(a.ctr + kCtrShares + sg.claim0 + m, kShareDone + part, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ bool claim_share(const PersistArgs& a, const PSeg& sg, uint32_t m) {
@@ -440,8 +448,56 @@ constexpr uint32_t kMaxNap = 16;            // x 512 clocks: about 4 us between 
 constexpr uint32_t kMaxWideNap = 1;         // level hand-overs: looks at most ~0.2 us apart
 constexpr uint64_t kCheckTicks = 1000;      // failure / abort words read at most every 10 us
 
-// First wavefront only (all 64 lanes): one look at the level's k share words.
-// Returns kStepDone, kStepFail, kStepSpin, or the index (< k) of a share
+// Bitwise OR of a value over the 64 lanes of the calling wavefront.
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
+// First wavefront only (all 64 lanes): is level sg of a team of k members,
+// q shares each and nparts parts complete for a workgroup of part `part`?
+// Every share done and, for a span team, every part that ran shares of it
+// released; this workgroup releases its own part when it is the first to
+// claim the part's release word (its XCD's L2 written back, then the flag).
+// *any_free: a share of the level was seen unclaimed.
+__device__ __forceinline__ bool level_complete(const PersistArgs& a, const PSeg& sg, uint32_t k, uint32_t q,
+                                               uint32_t nparts, uint32_t part, bool* any_free) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t ns = k * q;
+  uint32_t* w = a.ctr + kCtrShares + sg.claim0;
+  bool mine_done = true, mine_free = false;
+  uint32_t mask = 0;
+  for (uint32_t j = lane; j < ns; j += 64) {
+    const uint32_t v = ld_agent(w + j);
+    mine_done = mine_done && v >= kShareDone;
+    mine_free = mine_free || v == kShareFree;
+    if (v >= kShareDone) mask |= 1u << ((v - kShareDone) & 31u);
+  }
+  *any_free = __ballot(mine_free) != 0ull;
+  if (!__all(mine_done)) return false;
+  if (nparts < 2) return true;
+  mask = wave_or(mask);
+  uint32_t* rel = w + ns;  // the level's release words, one per part
+  if ((mask >> part) & 1u) {
+    uint32_t won = 0;
+    if (lane == 0 && ld_agent(rel + part) == kRelNone) {
+      uint32_t expect = kRelNone;
+      won = __hip_atomic_compare_exchange_strong(rel + part, &expect, kRelClaimed, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                ? 1u
+                : 0u;
+    }
+    if (lane == 0 && won) release_store(rel + part, kRelDone);  // the part's stores: from its L2 to memory
+  }
+  bool ok = true;
+  if (lane < nparts) ok = ((mask >> lane) & 1u) == 0u || ld_agent(rel + lane) == kRelDone;
+  return __all(ok);
+}
+
+// First wavefront only (all 64 lanes): one look at the level of team tm
+// (this workgroup of part `part`).
+// Returns kStepDone, kStepFail, kStepSpin, or the index (< k q) of a share
 // this workgroup has just claimed (steal).  Every decision is made by lane 0
 // and broadcast, so the wavefront's control flow stays uniform.  The
 // launch-wide failure word is read at most every kCheckTicks, the host's
@@ -449,18 +505,13 @@ constexpr uint64_t kCheckTicks = 1000;      // failure / abort words read at mos
 // workgroups (one per team) at that rate, which turn it into the failure
 // word; the sleep between looks grows, so the waiting workgroups put little
 // traffic on the memory system.
-__device__ __forceinline__ uint32_t wait_step(const PersistArgs& a, const PSeg& sg, uint32_t k, bool watch_host,
-                                              WaitState* ws) {
+__device__ __forceinline__ uint32_t wait_step(const PersistArgs& a, const PSeg& sg, const PTeam& tm, uint32_t part,
+                                              bool watch_host, WaitState* ws) {
   const uint32_t lane = threadIdx.x;
+  const uint32_t k = tm.k * tm.q;  // shares of the level
   const uint32_t* w = a.ctr + kCtrShares + sg.claim0;
-  bool mine_done = true, mine_free = false;
-  for (uint32_t j = lane; j < k; j += 64) {
-    const uint32_t v = ld_agent(w + j);
-    mine_done = mine_done && v == kShareDone;
-    mine_free = mine_free || v == kShareFree;
-  }
-  if (__all(mine_done)) return kStepDone;
-  const bool any_free = __ballot(mine_free) != 0ull;
+  bool any_free = false;
+  if (level_complete(a, sg, tm.k, tm.q, tm.nparts, part, &any_free)) return kStepDone;
   uint32_t out = kStepSpin;
   if (lane == 0) {
     const uint64_t now = wall_clock64();
@@ -505,15 +556,16 @@ __device__ __forceinline__ uint32_t wait_step(const PersistArgs& a, const PSeg& 
 }
 
 // First wavefront only (all 64 lanes): one look for work among the teams of
-// domain `dom` (the first 64 of its list) whose first member has arrived.
-// Lane i keeps team i's current level (the first whose shares are not all
-// done) in *lvl between looks.  Returns kStepDone (every such team is
-// complete), kStepFail, kStepSpin (nothing free now), or kStepRun with
-// *team, *seg and *share set: a share of a current level just claimed.
+// domain `dom` (the first 64 of its list) whose first member in the domain
+// (of the team's part there) has arrived.  Lane i keeps team i's current
+// level (the first not complete, see level_complete) in *lvl between looks.
+// Returns kStepDone (every such team is complete), kStepFail, kStepSpin
+// (nothing free now), or kStepRun with *team, *seg, *share and *part set: a
+// share of a current level just claimed.
 // Timeouts and the failure word are handled as in wait_step; the host's
 // abort word is left to the teams' first members.
 __device__ __forceinline__ uint32_t help_step(const PersistArgs& a, uint32_t dom, uint32_t* lvl, WaitState* ws,
-                                              uint32_t* team, uint32_t* seg, uint32_t* share) {
+                                              uint32_t* team, uint32_t* seg, uint32_t* share, uint32_t* part) {
   const uint32_t lane = threadIdx.x;
   const uint32_t t0 = a.dom_off[dom], nt = min(a.dom_off[dom + 1] - t0, 64u);
   const uint32_t arrived = ld_agent(a.ctr + kCtrArrive + dom);
@@ -521,23 +573,23 @@ __device__ __forceinline__ uint32_t help_step(const PersistArgs& a, uint32_t dom
   for (uint32_t i = 0; i < nt; ++i) {
     const uint32_t t = a.dom_teams[t0 + i];
     const PTeam tm = a.team[t];
-    if (arrived <= tm.slot0) continue;  // no member yet: possibly an orphan, run whole at the end
+    uint32_t pi = 0;  // the team's part in this domain
+    while (pi + 1 < tm.nparts && a.parts[tm.part0 + pi].dom != dom) ++pi;
+    if (arrived <= a.parts[tm.part0 + pi].slot0) continue;  // no member yet: possibly an orphan, run whole at the end
     const uint32_t ns = tm.k * tm.q;
     uint32_t cur = __shfl(*lvl, (int)i);
     while (cur < tm.nseg) {
       const PSeg sg = a.seg[tm.seg0 + cur];
       uint32_t* w = a.ctr + kCtrShares + sg.claim0;
-      bool mine_done = true;
-      uint32_t free_j = kStepDone;
-      for (uint32_t j = lane; j < ns; j += 64) {
-        const uint32_t v = ld_agent(w + j);
-        mine_done = mine_done && v == kShareDone;
-        if (v == kShareFree && free_j == kStepDone) free_j = j;
-      }
-      if (__all(mine_done)) {
+      bool any_free = false;
+      if (level_complete(a, sg, tm.k, tm.q, tm.nparts, pi, &any_free)) {
         ++cur;
         continue;
       }
+      if (!any_free) break;  // the level is under way (or handing over) with nothing free
+      uint32_t free_j = kStepDone;
+      for (uint32_t j = lane; j < ns; j += 64)
+        if (free_j == kStepDone && ld_agent(w + j) == kShareFree) free_j = j;
       uint64_t m = __ballot(free_j != kStepDone);
       while (m) {
         const int b = __ffsll((long long)m) - 1;
@@ -556,6 +608,7 @@ __device__ __forceinline__ uint32_t help_step(const PersistArgs& a, uint32_t dom
           *team = t;
           *seg = tm.seg0 + cur;
           *share = j;
+          *part = pi;
           ws->nap = 1;
           ws->t0 = wall_clock64();
           return kStepRun;
@@ -600,24 +653,25 @@ __device__ __forceinline__ uint32_t help_step(const PersistArgs& a, uint32_t dom
 template <int S>
 __device__ __forceinline__ bool help_teams(const PersistArgs& a, uint32_t dom, cx<float>* smem,
                                                      uint32_t* sh) {
-  uint32_t *s_step = sh, *s_team = sh + 3, *s_seg = sh + 4, *s_share = sh + 5;
+  uint32_t *s_step = sh, *s_team = sh + 3, *s_seg = sh + 4, *s_share = sh + 5, *s_part = sh + 7;
   const uint32_t tid = threadIdx.x;
   uint32_t lvl = 0;  // first wavefront: lane i, team i's current level
   WaitState ws{0, 0, 0, 1, 0, 0};
   if (tid < 64) ws.t0 = wall_clock64();
   for (;;) {
     if (tid < 64) {
-      uint32_t ht = 0, hs = 0, hj = 0;
-      const uint32_t st = help_step(a, dom, &lvl, &ws, &ht, &hs, &hj);
+      uint32_t ht = 0, hs = 0, hj = 0, hp = 0;
+      const uint32_t st = help_step(a, dom, &lvl, &ws, &ht, &hs, &hj, &hp);
       if (tid == 0) {
         *s_step = st;
         *s_team = ht;
         *s_seg = hs;
         *s_share = hj;
+        *s_part = hp;
       }
     }
     __syncthreads();
-    const uint32_t st = *s_step, ht = *s_team, hs = *s_seg, hj = *s_share;
+    const uint32_t st = *s_step, ht = *s_team, hs = *s_seg, hj = *s_share, hp = *s_part;
     __syncthreads();  // read before the next look writes them
     if (st == kStepDone) return false;
     if (st == kStepFail) return true;
@@ -626,7 +680,7 @@ __device__ __forceinline__ bool help_teams(const PersistArgs& a, uint32_t dom, c
       const PTeam tm = a.team[ht];
       const BXform x = a.xf[ht];
       const PSeg sg = a.seg[hs];
-      run_share<S>(a, x, sg, tm.k * tm.q, hj, smem);
+      run_share<S>(a, x, sg, tm.k * tm.q, hj, hp, smem);
     }
   }
 }
@@ -642,32 +696,34 @@ __device__ void run_whole(const PersistArgs& a, uint32_t t, cx<float>* smem) {
   }
 }
 
-// The kernel's body; sh: seven LDS words (step, last, claimed, team, member,
-// job, domain).
+// The kernel's body; sh: eight LDS words (step, last, claimed, team, member,
+// job, domain, part).
 template <int S>
 __device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* smem, uint32_t* sh) {
   uint32_t *s_step = sh, *s_last = sh + 1, *s_claimed = sh + 2, *s_team = sh + 3, *s_member = sh + 4,
-           *s_job = sh + 5, *s_dom = sh + 6;
+           *s_job = sh + 5, *s_dom = sh + 6, *s_part = sh + 7;
   const uint32_t tid = threadIdx.x;
   bool failed = false;
   // the next local slot of this workgroup's domain
   if (tid == 0) {
     const uint32_t d = xcc_id();
-    uint32_t team = kNoTeam, member = 0;
+    uint32_t team = kNoTeam, member = 0, part = 0;
     if (d < a.domains) {
       const uint32_t li = __hip_atomic_fetch_add(a.ctr + kCtrArrive + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (li < a.per_dom) {
         const PEntry e = a.local[d * a.per_dom + li];
         team = e.team;
         member = e.member;
+        part = e.part;
       }
     }
     *s_team = team;
     *s_member = member;
     *s_dom = d;
+    *s_part = part;
   }
   __syncthreads();
-  const uint32_t team = *s_team, member = *s_member, dom = *s_dom;
+  const uint32_t team = *s_team, member = *s_member, dom = *s_dom, part = *s_part;
   if (team != kNoTeam) {  // a team member: per level the own share, then wait (stealing) for the level
     const PTeam tm = a.team[team];
     const BXform x = a.xf[team];
@@ -680,13 +736,13 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* sm
         __syncthreads();
         const bool mine = *s_claimed != 0u;
         __syncthreads();  // s_claimed is read before the next claim writes it
-        if (mine) run_share<S>(a, x, sg, ns, own, smem);
+        if (mine) run_share<S>(a, x, sg, ns, own, part, smem);
       }
       WaitState ws{0, 0, ns - 1 - member, 1, 0, 0};
       if (tid < 64) ws.t0 = wall_clock64();  // checked = 0: the first look reads the launch-wide words
       for (;;) {
         if (tid < 64) {
-          const uint32_t st = wait_step(a, sg, ns, member == 0, &ws);
+          const uint32_t st = wait_step(a, sg, tm, part, member == 0, &ws);
           if (tid == 0) *s_step = st;
         }
         __syncthreads();
@@ -697,7 +753,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* sm
           failed = true;
           break;
         }
-        if (st < ns) run_share<S>(a, x, sg, ns, st, smem);  // a stolen share
+        if (st < ns) run_share<S>(a, x, sg, ns, st, part, smem);  // a stolen share
       }
       if (!failed) acquire_level();  // the level's stores, from the L2
     }
@@ -726,7 +782,12 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, cx<float>* sm
       for (uint32_t i = 0; i < a.nteams; ++i) {
         const uint32_t t = a.team_ids[i];
         const PTeam tm = a.team[t];
-        if (ld_agent(a.ctr + kCtrArrive + tm.dom) <= tm.slot0) run_whole<S>(a, t, smem);  // no member arrived
+        bool orphan = true;  // no member of any part arrived
+        for (uint32_t i = 0; i < tm.nparts; ++i) {
+          const PPart pt = a.parts[tm.part0 + i];
+          orphan = orphan && ld_agent(a.ctr + kCtrArrive + pt.dom) <= pt.slot0;
+        }
+        if (orphan) run_whole<S>(a, t, smem);
       }
     if (tid == 0) {
       const uint32_t f = ld_agent(a.ctr + kCtrFail);

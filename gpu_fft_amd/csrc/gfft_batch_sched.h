@@ -24,19 +24,33 @@
 // Teams are placed on domains longest first, onto the least-loaded domain
 // with room.
 //
+// Span teams.  A transform whose share of members exceeds a domain's P slots
+// (the largest transforms of a batch) gets a team split into n = ceil(k / P)
+// *parts* of near-equal size, each on its own domain (the least-loaded with
+// room).  Its level hand-over crosses L2s: a share's done word records the
+// part of the workgroup that ran it (kShareDone + part); once every share of
+// a level is done, for each part that ran shares one workgroup of that part
+// (the first to claim the part's release word) writes its XCD's L2 back
+// (agent-scope release) and marks the part released; the team's workgroups
+// go on to the next level (agent-scope acquire) once every such part has
+// released.  A part ran shares only if one of its workgroups arrived, and
+// every arrived member walks every level, so each needed release happens.
+//
 // Solo transforms (the rest) form one queue, longest first.  Every
 // workgroup, once its team work (if any) is over, takes solo transforms from
 // the queue (one atomic ticket each) and runs each whole, level after level,
 // with nothing shared outside the workgroup.
 //
 // Helpers.  A workgroup that finds the solo queue empty helps the teams of
-// its own domain (same L2, so the L2-level hand-over holds for it as for a
-// member): for each team of the domain whose first member has arrived, it
-// finds the current level (the first whose shares are not all done: every
-// earlier level is complete), claims a free share of it (the same compare-
-// and-swap as a stealing member), runs it and marks it done; it leaves when
-// every such team is complete.  Teams of which no member has arrived are
-// left alone, so the orphan rule below still finds them untouched.
+// its own domain (same L2 as the team's part there, so the hand-over holds
+// for it as for a member of that part): for each team with a part in the
+// domain whose first member has arrived, it finds the current level (the
+// first not complete: not all shares done, or, for a span team, a part that
+// ran shares not yet released; it releases its own part when that is what
+// is missing), claims a free share of it (the same compare-and-swap as a
+// stealing member), runs it and marks it done; it leaves when every such
+// team is complete.  Teams of which no member has arrived are left alone, so
+// the orphan rule below still finds them untouched.
 //
 // Team synchronisation, per level: every share has a state word (0 =
 // unclaimed, 1 = claimed, 2 = done).  A member claims each of its shares in
@@ -52,9 +66,10 @@
 // member of the team is running, every share of the level gets run (by its
 // owner or by a waiting member that steals it), for any number of resident
 // workgroups; members that start late find their shares taken or done and
-// walk through the levels.  A team of which no member arrives (its domain
-// received fewer workgroups than slot0 + 1, or none) is *orphaned*: the last
-// workgroup to leave finds it from the arrival counters and runs it whole.
+// walk through the levels.  A team of which no member arrives (none of its
+// parts' domains received more workgroups than the part's first slot) is
+// *orphaned*: the last workgroup to leave finds it from the arrival counters
+// and runs it whole.
 // Solo transforms wait for nobody, and helpers wait only while a team they
 // may help is incomplete, which ends as above.  Hence the launch completes,
 // with every tile run exactly once, whatever the residency and whatever the
@@ -89,16 +104,24 @@ struct PTeam {            // one transform
   uint32_t seg0, nseg;    // its non-empty levels, in order
   uint32_t k;             // members (1 = solo)
   uint32_t q;             // shares per member (teams)
-  uint32_t dom, slot0;    // teams: domain, first local slot (members slot0 .. slot0 + k - 1)
+  uint32_t dom, slot0;    // teams: part 0's domain and first local slot
+  uint32_t nparts;        // teams: domains spanned (1: members slot0 .. slot0 + k - 1 of dom)
+  uint32_t part0;         // teams: its parts are parts[part0 .. part0 + nparts)
 };
 
-struct PEntry {           // what a local slot runs: member `member` of team `team` (kNoTeam: none)
-  uint32_t team, member;
+struct PPart {            // the members of a team on one domain
+  uint32_t dom, slot0;    // domain, first local slot
+  uint32_t k, m0;         // members m0 .. m0 + k - 1 on local slots slot0 .. slot0 + k - 1
+};
+
+struct PEntry {           // what a local slot runs: member `member` (of part `part`) of team `team`
+  uint32_t team, member, part;  // (team kNoTeam: none)
 };
 
 struct PPlan {
   std::vector<PSeg> segs;
   std::vector<PTeam> teams;     // one per transform
+  std::vector<PPart> parts;     // of every team (one for a team in one domain)
   uint32_t domains = 1, per_dom = 1;
   std::vector<PEntry> local;    // domains * per_dom local slots, domain-major
   std::vector<uint32_t> solo;   // solo transforms, longest first
@@ -121,26 +144,32 @@ GFFT_HD inline uint32_t share_tiles(uint32_t ntiles, uint32_t n, uint32_t s) {
 // Counter words of one launch (device memory, zero between launches): the
 // count of workgroups that left, the launch's failure bits (kFail*), a
 // sticky status word, the solo queue's ticket, one arrival counter per
-// domain, then the share state words of every team level.  The last
+// domain, then the share state words of every team level (a span team's
+// level: its k q share words, then one release word per part).  The last
 // workgroup to leave ORs the failure bits into the status word (which only
 // the host clears) and zeroes every other word.
 constexpr uint32_t kCtrExited = 0, kCtrFail = 1, kCtrStatus = 2, kCtrSolo = 3, kCtrArrive = 4,
                    kCtrShares = kCtrArrive + kMaxDomains;
 constexpr uint32_t kFailTimeout = 1, kFailAborted = 2;
-constexpr uint32_t kShareFree = 0, kShareClaimed = 1, kShareDone = 2;
+constexpr uint32_t kShareFree = 0, kShareClaimed = 1, kShareDone = 2;  // done: kShareDone + part
+constexpr uint32_t kRelNone = 0, kRelClaimed = 1, kRelDone = 2;        // a span team part's release word
+// claim words of one level of a team
+GFFT_HD inline uint32_t level_words(uint32_t k, uint32_t q, uint32_t nparts) {
+  return k * q + (nparts > 1 ? nparts : 0u);
+}
 inline size_t counter_words(const PPlan& p) { return kCtrShares + p.claim_words; }
 
 // tiles(t, l), cls(t, l): callables; work(t): the transform's relative
 // cost; domains D and local slots per domain P.  A transform gets a team
 // when its work exceeds team_factor slot shares (share = total work / (D P)),
-// of about work / share members (at most P); min_team_tiles: a member owns
-// at least this many tiles of the transform's smallest level; each member
-// owns q = shares_per_member shares of a level (fewer where a level has
-// too few tiles).
+// of about work / share members (at most P, or max_parts P split over that
+// many domains); min_team_tiles: a member owns at least this many tiles of
+// the transform's smallest level; each member owns q = shares_per_member
+// shares of a level (fewer where a level has too few tiles).
 template <typename TilesFn, typename ClsFn, typename WorkFn>
 PPlan build_plan(uint32_t ntransforms, int nlevels, const PLevelSpec* levels, TilesFn tiles, ClsFn cls,
                  WorkFn work, uint32_t domains, uint32_t per_dom, double team_factor = 2.0,
-                 uint32_t min_team_tiles = 2, uint32_t shares_per_member = 2) {
+                 uint32_t min_team_tiles = 2, uint32_t shares_per_member = 2, uint32_t max_parts = 2) {
   PPlan p;
   domains = std::min(std::max(domains, 1u), kMaxDomains);
   per_dom = std::max(per_dom, 1u);
@@ -154,10 +183,13 @@ PPlan build_plan(uint32_t ntransforms, int nlevels, const PLevelSpec* levels, Ti
   for (uint32_t t = 0; t < ntransforms; ++t) order[t] = t;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return w[a] > w[b]; });
   // team sizes (about one slot share of work per member), then domains:
-  // longest first onto the least-loaded domain that has room
-  std::vector<uint32_t> k(ntransforms, 1), q(ntransforms, 1), dom(ntransforms, 0), slot0(ntransforms, 0);
+  // longest first onto the least-loaded domain that has room; a team of more
+  // than P members splits into near-equal parts on as many domains
+  std::vector<uint32_t> k(ntransforms, 1), q(ntransforms, 1);
+  std::vector<std::vector<PPart>> tp(ntransforms);
   std::vector<uint32_t> used(domains, 0);
   std::vector<double> load(domains, 0.0);
+  const uint32_t span = std::max(1u, std::min(max_parts, domains));
   for (uint32_t t : order) {
     uint32_t min_tiles = 0xffffffffu;
     for (int l = 0; l < nlevels; ++l) {
@@ -167,34 +199,62 @@ PPlan build_plan(uint32_t ntransforms, int nlevels, const PLevelSpec* levels, Ti
     if (min_tiles == 0xffffffffu) min_tiles = 0;
     uint32_t kk = w[t] > team_factor * share ? (uint32_t)(w[t] / share + 0.5) : 1u;
     kk = std::min(kk, std::max(1u, min_tiles / std::max(1u, min_team_tiles)));
-    kk = std::min(kk, per_dom);
+    kk = std::min(kk, per_dom * span);
     if (kk < 2) continue;
-    uint32_t best = domains;
-    for (uint32_t d = 0; d < domains; ++d)
-      if (used[d] + kk <= per_dom && (best == domains || load[d] < load[best])) best = d;
-    if (best == domains) {  // no domain has room for kk: the roomiest one, with a smaller team
-      best = 0;
-      for (uint32_t d = 1; d < domains; ++d)
-        if (used[d] < used[best]) best = d;
-      kk = per_dom - used[best];
-      if (kk < 2) continue;
+    std::vector<PPart> parts;
+    const uint32_t np = (kk + per_dom - 1) / per_dom;
+    if (np > 1) {
+      std::vector<char> taken(domains, 0);
+      uint32_t m0 = 0;
+      for (uint32_t i = 0; i < np; ++i) {
+        const uint32_t kp = kk / np + (i < kk % np ? 1u : 0u);
+        uint32_t best = domains;
+        for (uint32_t d = 0; d < domains; ++d)
+          if (!taken[d] && used[d] + kp <= per_dom && (best == domains || load[d] < load[best])) best = d;
+        if (best == domains) break;
+        taken[best] = 1;
+        parts.push_back(PPart{best, used[best], kp, m0});
+        m0 += kp;
+      }
+      if (parts.size() != np) {  // no room for the parts: one domain, as below
+        parts.clear();
+        kk = per_dom;
+      }
+    }
+    if (parts.empty()) {
+      uint32_t best = domains;
+      for (uint32_t d = 0; d < domains; ++d)
+        if (used[d] + kk <= per_dom && (best == domains || load[d] < load[best])) best = d;
+      if (best == domains) {  // no domain has room for kk: the roomiest one, with a smaller team
+        best = 0;
+        for (uint32_t d = 1; d < domains; ++d)
+          if (used[d] < used[best]) best = d;
+        kk = per_dom - used[best];
+        if (kk < 2) continue;
+      }
+      parts.push_back(PPart{best, used[best], kk, 0u});
     }
     k[t] = kk;
     q[t] = std::max(1u, std::min(std::max(1u, shares_per_member), min_tiles / kk));
-    dom[t] = best;
-    slot0[t] = used[best];
-    used[best] += kk;
-    load[best] += w[t];
+    for (const PPart& pt : parts) {
+      used[pt.dom] += pt.k;
+      load[pt.dom] += w[t] * pt.k / kk;
+    }
+    tp[t] = parts;
   }
   // levels and teams
   p.teams.resize(ntransforms);
   for (uint32_t t = 0; t < ntransforms; ++t) {
     PTeam& tm = p.teams[t];
+    const bool team = k[t] >= 2;
     tm.seg0 = (uint32_t)p.segs.size();
     tm.k = k[t];
-    tm.q = k[t] >= 2 ? q[t] : 1u;
-    tm.dom = k[t] >= 2 ? dom[t] : 0u;
-    tm.slot0 = k[t] >= 2 ? slot0[t] : 0u;
+    tm.q = team ? q[t] : 1u;
+    tm.dom = team ? tp[t][0].dom : 0u;
+    tm.slot0 = team ? tp[t][0].slot0 : 0u;
+    tm.nparts = team ? (uint32_t)tp[t].size() : 0u;
+    tm.part0 = (uint32_t)p.parts.size();
+    if (team) p.parts.insert(p.parts.end(), tp[t].begin(), tp[t].end());
     for (int l = 0; l < nlevels; ++l) {
       const uint32_t n = tiles(t, l);
       if (!n) continue;
@@ -204,29 +264,35 @@ PPlan build_plan(uint32_t ntransforms, int nlevels, const PLevelSpec* levels, Ti
       s.inv = levels[l].inv;
       s.cls = cls(t, l);
       s.ntiles = n;
-      s.claim0 = tm.k >= 2 ? p.claim_words : 0u;
-      if (tm.k >= 2) p.claim_words += tm.k * tm.q;
+      s.claim0 = team ? p.claim_words : 0u;
+      if (team) p.claim_words += level_words(tm.k, tm.q, tm.nparts);
       p.segs.push_back(s);
     }
     tm.nseg = (uint32_t)p.segs.size() - tm.seg0;
   }
   // local slots of the members; the solo queue, longest first
-  p.local.assign((size_t)domains * per_dom, PEntry{kNoTeam, 0});
+  p.local.assign((size_t)domains * per_dom, PEntry{kNoTeam, 0, 0});
   for (uint32_t t : order) {
     if (k[t] >= 2) {
       p.team_ids.push_back(t);
-      for (uint32_t m = 0; m < k[t]; ++m) p.local[(size_t)dom[t] * per_dom + slot0[t] + m] = PEntry{t, m};
+      for (uint32_t i = 0; i < (uint32_t)tp[t].size(); ++i) {
+        const PPart& pt = tp[t][i];
+        for (uint32_t m = 0; m < pt.k; ++m) p.local[(size_t)pt.dom * per_dom + pt.slot0 + m] = PEntry{t, pt.m0 + m, i};
+      }
     } else if (p.teams[t].nseg > 0) {
       p.solo.push_back(t);
     }
   }
-  // each domain's teams for its helpers, longest first (team_ids' order)
+  // each domain's teams for its helpers (a span team under each of its
+  // parts' domains), longest first (team_ids' order)
   p.dom_off.assign(domains + 1, 0u);
-  for (uint32_t t : p.team_ids) p.dom_off[dom[t] + 1]++;
+  for (uint32_t t : p.team_ids)
+    for (const PPart& pt : tp[t]) p.dom_off[pt.dom + 1]++;
   for (uint32_t d = 0; d < domains; ++d) p.dom_off[d + 1] += p.dom_off[d];
-  p.dom_teams.resize(p.team_ids.size());
+  p.dom_teams.resize(p.dom_off.back());
   std::vector<uint32_t> fill(p.dom_off.begin(), p.dom_off.end() - 1);
-  for (uint32_t t : p.team_ids) p.dom_teams[fill[dom[t]]++] = t;
+  for (uint32_t t : p.team_ids)
+    for (const PPart& pt : tp[t]) p.dom_teams[fill[pt.dom]++] = t;
   return p;
 }
 
@@ -238,14 +304,30 @@ PPlan build_plan(uint32_t ntransforms, int nlevels, const PLevelSpec* levels, Ti
 // Returns 0, or a positive code.
 inline int check_plan(const PPlan& p) {
   if (p.domains < 1 || p.domains > kMaxDomains || p.local.size() != (size_t)p.domains * p.per_dom) return 1;
+  // parts: members m0 .. m0 + k - 1 in order, distinct domains, part 0 = (dom, slot0)
+  for (const PTeam& tm : p.teams) {
+    if (tm.k < 2) continue;
+    if (tm.nparts < 1 || tm.nparts > p.domains || (size_t)tm.part0 + tm.nparts > p.parts.size()) return 12;
+    uint32_t m0 = 0, doms = 0;
+    for (uint32_t i = 0; i < tm.nparts; ++i) {
+      const PPart& pt = p.parts[tm.part0 + i];
+      if (pt.dom >= p.domains || pt.k < 1 || pt.m0 != m0 || pt.slot0 + pt.k > p.per_dom || (doms >> pt.dom) & 1u)
+        return 12;
+      doms |= 1u << pt.dom;
+      m0 += pt.k;
+    }
+    if (m0 != tm.k || p.parts[tm.part0].dom != tm.dom || p.parts[tm.part0].slot0 != tm.slot0) return 12;
+  }
   std::vector<int> seen(p.teams.size(), 0);
   for (size_t i = 0; i < p.local.size(); ++i) {
     const PEntry& e = p.local[i];
     if (e.team == kNoTeam) continue;
     if (e.team >= p.teams.size()) return 2;
     const PTeam& tm = p.teams[e.team];
-    if (tm.k < 2 || e.member >= tm.k) return 2;
-    if (i != (size_t)tm.dom * p.per_dom + tm.slot0 + e.member) return 3;
+    if (tm.k < 2 || e.member >= tm.k || e.part >= tm.nparts) return 2;
+    const PPart& pt = p.parts[tm.part0 + e.part];
+    if (e.member < pt.m0 || e.member >= pt.m0 + pt.k) return 3;
+    if (i != (size_t)pt.dom * p.per_dom + pt.slot0 + (e.member - pt.m0)) return 3;
     seen[e.team]++;
   }
   for (uint32_t t : p.solo) {
@@ -270,21 +352,28 @@ inline int check_plan(const PPlan& p) {
       for (uint32_t j = 0; j < n; ++j) sum += share_tiles(sg.ntiles, n, j);
       if (sum != sg.ntiles) return 7;
       if (tm.k >= 2)
-        for (uint32_t j = 0; j < n; ++j) {
+        for (uint32_t j = 0; j < level_words(tm.k, tm.q, tm.nparts); ++j) {
           if (sg.claim0 + j >= p.claim_words) return 8;
           if (claim[sg.claim0 + j]++) return 9;
         }
     }
   }
-  if (p.dom_off.size() != (size_t)p.domains + 1 || p.dom_off[0] != 0 || p.dom_off.back() != p.dom_teams.size() ||
-      p.dom_teams.size() != p.team_ids.size())
+  // the helpers' lists: every team once under each of its parts' domains
+  if (p.dom_off.size() != (size_t)p.domains + 1 || p.dom_off[0] != 0 || p.dom_off.back() != p.dom_teams.size())
     return 10;
-  std::vector<int> listed(p.teams.size(), 0);
+  size_t want = 0;
+  for (uint32_t t : p.team_ids) want += p.teams[t].nparts;
+  if (p.dom_teams.size() != want) return 10;
+  std::vector<uint32_t> listed(p.teams.size(), 0);  // bit d: listed under domain d
   for (uint32_t d = 0; d < p.domains; ++d) {
     if (p.dom_off[d] > p.dom_off[d + 1]) return 10;
     for (uint32_t i = p.dom_off[d]; i < p.dom_off[d + 1]; ++i) {
       const uint32_t t = p.dom_teams[i];
-      if (t >= p.teams.size() || p.teams[t].k < 2 || p.teams[t].dom != d || listed[t]++) return 11;
+      if (t >= p.teams.size() || p.teams[t].k < 2 || (listed[t] >> d) & 1u) return 11;
+      bool has = false;
+      for (uint32_t j = 0; j < p.teams[t].nparts; ++j) has = has || p.parts[p.teams[t].part0 + j].dom == d;
+      if (!has) return 11;
+      listed[t] |= 1u << d;
     }
   }
   return 0;

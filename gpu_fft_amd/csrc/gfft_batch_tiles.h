@@ -265,6 +265,7 @@ struct PersistArgs {
   const BXform* xf;            // descriptors, batch order (= team order)
   const PSeg* seg;             // levels of every transform
   const PTeam* team;           // one per transform
+  const PPart* parts;          // the teams' parts (a span team has several, on distinct domains)
   const PEntry* local;         // domains * per_dom local slots, domain-major
   const uint32_t* solo;        // solo queue (transforms run whole by one workgroup)
   const uint32_t* team_ids;    // transforms run by teams

@@ -229,7 +229,9 @@ int gfft_query_plan(int64_t nx, int64_t ny, int64_t nz, int procs, int rank, int
  * runs every axis pass of every transform, one workgroup per slot of a plan
  * made at creation: small transforms are run whole by one workgroup each
  * (several per workgroup, packed longest first), larger ones by a team of
- * workgroups whose members own interleaved tile shares of each pass.  A
+ * workgroups whose members own interleaved tile shares of each pass (the
+ * largest by a team spanning XCDs, see gfft_batch_set_span); a workgroup
+ * with nothing left helps its XCD's teams.  A
  * team member waits only for the shares of its own transform's pass (per
  * share state words, no grid-wide barrier) and takes over shares whose owner
  * has not started, so completion never depends on how many workgroups are
@@ -322,9 +324,18 @@ int gfft_batch_scheduler(gfft_batch batch, int* scheduler);
  * smallest transforms run whole from a queue meanwhile); query them and (if
  * variants is not NULL) the set's variant per size class, 8 ints for the
  * sizes 32 ... 256.  set rebuilds the plans and synchronises the device.
- * gfft_batch_tune tries team factors 0, 1, 2 and 4 with both sets. */
+ * gfft_batch_tune tries team factors 0, 1, 2 and 4 with both sets, and
+ * factors 1 and 2 with teams kept to one XCD (span 1). */
 int gfft_batch_set_persist(gfft_batch batch, int set, int team_factor);
 int gfft_batch_persist(gfft_batch batch, int* set, int* team_factor, int* variants);
+
+/* Team plans: the most XCDs one team may span (1..8, default 2).  A
+ * transform whose share of workgroups exceeds one XCD's slots gets a team
+ * split over that many XCDs, whose members hand each axis pass over through
+ * an L2 write-back per XCD part.  set rebuilds the plans and synchronises
+ * the device. */
+int gfft_batch_set_span(gfft_batch batch, int parts);
+int gfft_batch_span(gfft_batch batch, int* parts);
 
 /* The persistent plan of `mode`: slots (workgroups of the launch), teams
  * (transforms run by two or more workgroups) and their members (global-level

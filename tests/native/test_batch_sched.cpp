@@ -127,7 +127,7 @@ enum class Spread { RoundRobin, AllInZero, Random, HalfOutside };
 
 static SimResult simulate(const gfft::PPlan& p, Ctr& ctr, int resident, bool steal,
                           std::chrono::milliseconds wait_limit, unsigned seed, Spread spread,
-                          bool help_checks_arrival = true) {
+                          bool help_checks_arrival = true, bool release_parts = true) {
   const size_t nseg = p.segs.size();
   std::vector<std::vector<std::atomic<int>>> seen(nseg);
   std::vector<std::atomic<uint32_t>> finished(nseg);  // test-only: tiles actually run per level
@@ -140,6 +140,8 @@ static SimResult simulate(const gfft::PPlan& p, Ctr& ctr, int resident, bool ste
   Slots places(resident);
   const int groups = (int)(p.domains * p.per_dom);
   auto failed = [&] { return ctr[gfft::kCtrFail].load(std::memory_order_relaxed) != 0; };
+  auto words = [&](uint32_t s) { return &ctr[gfft::kCtrShares + p.segs[s].claim0]; };
+  auto part_dom = [&](const gfft::PTeam& tm, uint32_t i) { return p.parts[tm.part0 + i].dom; };
   auto run_tiles = [&](uint32_t s, uint32_t k, uint32_t m, std::mt19937& rng) {
     const gfft::PSeg& sg = p.segs[s];
     const gfft::PTeam& tm = p.teams[sg.xf];
@@ -148,50 +150,87 @@ static SimResult simulate(const gfft::PPlan& p, Ctr& ctr, int resident, bool ste
     finished[s].fetch_add(gfft::share_tiles(sg.ntiles, k, m));  // k: shares of the level, m: the share
     if (rng() % 4 == 0) std::this_thread::sleep_for(std::chrono::microseconds(rng() % 40));
   };
+  // What the kernel's memory ordering needs before a team share of level s
+  // runs on domain d: the level before ran on d only, or (span teams) every
+  // other part that ran shares of it has released (written its L2 back).
+  auto prev_visible = [&](uint32_t s, uint32_t d) {
+    const gfft::PTeam& tm = p.teams[p.segs[s].xf];
+    if (s == tm.seg0) return true;
+    const uint32_t n = tm.k * tm.q;
+    std::atomic<uint32_t>* w = words(s - 1);
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint32_t v = w[j].load(std::memory_order_acquire);
+      if (v < gfft::kShareDone) return false;
+      const uint32_t q = v - gfft::kShareDone;
+      if (part_dom(tm, q) != d && (tm.nparts < 2 || w[n + q].load(std::memory_order_acquire) != gfft::kRelDone))
+        return false;
+    }
+    return true;
+  };
+  // level s of team tm complete for a workgroup of part `mine`: every share
+  // done and, for a span team, every part that ran shares released (its own
+  // part released by it when it is the first to find that missing)
+  auto level_complete = [&](const gfft::PTeam& tm, uint32_t s, uint32_t mine) {
+    const uint32_t n = tm.k * tm.q;
+    std::atomic<uint32_t>* w = words(s);
+    uint32_t mask = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint32_t v = w[j].load(std::memory_order_acquire);
+      if (v < gfft::kShareDone) return false;
+      mask |= 1u << (v - gfft::kShareDone);
+    }
+    if (tm.nparts < 2 || !release_parts) return true;
+    std::atomic<uint32_t>* rel = w + n;
+    if ((mask >> mine) & 1u) {
+      uint32_t z = gfft::kRelNone;
+      if (rel[mine].compare_exchange_strong(z, gfft::kRelClaimed))  // the write-back, then the flag
+        rel[mine].store(gfft::kRelDone, std::memory_order_release);
+    }
+    for (uint32_t q = 0; q < tm.nparts; ++q)
+      if (((mask >> q) & 1u) && rel[q].load(std::memory_order_acquire) != gfft::kRelDone) return false;
+    return true;
+  };
+  // a team share: run on domain d by a workgroup of part `part`
+  auto run_share = [&](uint32_t s, uint32_t j, uint32_t d, uint32_t part, std::mt19937& rng) {
+    const gfft::PTeam& tm = p.teams[p.segs[s].xf];
+    if (part_dom(tm, part) != d) domain_bad = true;  // a part's hand-over needs its own L2
+    if (!prev_visible(s, d)) order_bad = true;
+    run_tiles(s, tm.k * tm.q, j, rng);
+    words(s)[j].store(gfft::kShareDone + part, std::memory_order_release);
+  };
+  auto claim = [&](uint32_t s, uint32_t j) {
+    uint32_t z = gfft::kShareFree;
+    return words(s)[j].compare_exchange_strong(z, gfft::kShareClaimed);
+  };
   auto run_whole = [&](uint32_t t, std::mt19937& rng) {
     const gfft::PTeam& tm = p.teams[t];
     for (uint32_t s = tm.seg0; s < tm.seg0 + tm.nseg && !failed(); ++s) run_tiles(s, 1, 0, rng);
   };
   auto run_member = [&](const gfft::PEntry& e, uint32_t d, std::mt19937& rng) {
     const gfft::PTeam& tm = p.teams[e.team];
+    const uint32_t n = tm.k * tm.q;
     for (uint32_t s = tm.seg0; s < tm.seg0 + tm.nseg && !failed(); ++s) {
-      const gfft::PSeg& sg = p.segs[s];
-      std::atomic<uint32_t>* share = &ctr[gfft::kCtrShares + sg.claim0];
-      auto claim = [&](uint32_t m) {
-        uint32_t z = gfft::kShareFree;
-        return share[m].compare_exchange_strong(z, gfft::kShareClaimed);
-      };
-      const uint32_t n = tm.k * tm.q;
-      auto run_share = [&](uint32_t j) {
-        if (d != tm.dom) domain_bad = true;  // the kernel's light hand-over needs one L2
-        run_tiles(s, n, j, rng);
-        share[j].store(gfft::kShareDone, std::memory_order_release);
-      };
-      auto level_done = [&] {
-        for (uint32_t j = 0; j < n; ++j)
-          if (share[j].load(std::memory_order_acquire) != gfft::kShareDone) return false;
-        return true;
-      };
       for (uint32_t i = 0; i < tm.q; ++i)
-        if (claim(e.member + i * tm.k)) run_share(e.member + i * tm.k);
+        if (claim(s, e.member + i * tm.k)) run_share(s, e.member + i * tm.k, d, e.part, rng);
       const auto t0 = std::chrono::steady_clock::now();
       uint32_t spins = 0, next = (e.member + 1) % n;
-      while (!level_done()) {
+      while (!level_complete(tm, s, e.part)) {
         if (failed()) break;
         if (std::chrono::steady_clock::now() - t0 > wait_limit) {
           ctr[gfft::kCtrFail].fetch_or(gfft::kFailTimeout);
           break;
         }
         if (steal && ++spins > 20) {  // look at one other share per spin
-          if (claim(next)) run_share(next);
+          if (claim(s, next)) run_share(s, next, d, e.part, rng);
           next = (next + 1) % n;
         }
         std::this_thread::yield();
       }
     }
   };
-  // a helper: free shares of the current level of each team of domain d
-  // whose first member has arrived, until every such team is complete
+  // a helper: free shares of the current level of each team with a part in
+  // domain d whose first member there has arrived, until every such team is
+  // complete
   auto help = [&](uint32_t d, std::mt19937& rng) {
     auto t0 = std::chrono::steady_clock::now();
     for (;;) {
@@ -200,28 +239,22 @@ static SimResult simulate(const gfft::PPlan& p, Ctr& ctr, int resident, bool ste
       for (uint32_t i = p.dom_off[d]; i < p.dom_off[d + 1] && !ran; ++i) {
         const uint32_t t = p.dom_teams[i];
         const gfft::PTeam& tm = p.teams[t];
-        if (help_checks_arrival && ctr[gfft::kCtrArrive + d].load() <= tm.slot0) continue;
+        uint32_t part = 0;
+        while (part < tm.nparts && part_dom(tm, part) != d) ++part;
+        if (part == tm.nparts) domain_bad = true;  // listed under a domain it has no part in
+        if (help_checks_arrival &&
+            ctr[gfft::kCtrArrive + d].load() <= p.parts[tm.part0 + std::min(part, tm.nparts - 1)].slot0)
+          continue;
         const uint32_t n = tm.k * tm.q;
         uint32_t s = tm.seg0;
-        for (; s < tm.seg0 + tm.nseg; ++s) {
-          bool all = true;
-          for (uint32_t j = 0; j < n; ++j)
-            all = all && ctr[gfft::kCtrShares + p.segs[s].claim0 + j].load(std::memory_order_acquire) ==
-                             gfft::kShareDone;
-          if (!all) break;
-        }
+        while (s < tm.seg0 + tm.nseg && level_complete(tm, s, part)) ++s;
         if (s == tm.seg0 + tm.nseg) continue;  // complete
         unfinished = true;
-        std::atomic<uint32_t>* share = &ctr[gfft::kCtrShares + p.segs[s].claim0];
-        for (uint32_t j = 0; j < n && !ran; ++j) {
-          uint32_t z = gfft::kShareFree;
-          if (share[j].compare_exchange_strong(z, gfft::kShareClaimed)) {
-            if (d != tm.dom) domain_bad = true;
-            run_tiles(s, n, j, rng);
-            share[j].store(gfft::kShareDone, std::memory_order_release);
+        for (uint32_t j = 0; j < n && !ran; ++j)
+          if (claim(s, j)) {
+            run_share(s, j, d, part, rng);
             ran = true;
           }
-        }
       }
       if (!unfinished) return;
       if (ran) {
@@ -232,6 +265,13 @@ static SimResult simulate(const gfft::PPlan& p, Ctr& ctr, int resident, bool ste
       }
       std::this_thread::yield();
     }
+  };
+  auto orphaned = [&](const gfft::PTeam& tm) {
+    for (uint32_t i = 0; i < tm.nparts; ++i) {
+      const gfft::PPart& pt = p.parts[tm.part0 + i];
+      if (ctr[gfft::kCtrArrive + pt.dom].load() > pt.slot0) return false;
+    }
+    return true;
   };
   auto wg = [&](int w) {
     places.take();
@@ -257,7 +297,7 @@ static SimResult simulate(const gfft::PPlan& p, Ctr& ctr, int resident, bool ste
     const uint32_t prev = ctr[gfft::kCtrExited].fetch_add(1, std::memory_order_acq_rel);
     if (prev == (uint32_t)groups - 1) {  // last one out
       for (uint32_t t : p.team_ids)
-        if (!failed() && ctr[gfft::kCtrArrive + p.teams[t].dom].load() <= p.teams[t].slot0) run_whole(t, rng);
+        if (!failed() && orphaned(p.teams[t])) run_whole(t, rng);
       ctr[gfft::kCtrStatus].fetch_or(ctr[gfft::kCtrFail].load());
       for (size_t i = 0; i < ctr.size(); ++i)
         if (i != gfft::kCtrStatus) ctr[i].store(0);
@@ -291,7 +331,8 @@ static void plan_invariants_random() {
     for (const auto& s : p.segs) planned += s.ntiles;
     CHECK(tiles == planned);
   }
-  // the bench's shape of batch: the large transform gets a team in one domain
+  // the bench's shape of batch: the large transform's share of members
+  // exceeds a domain's slots, so its team spans two domains
   Batch b;
   b.tiles = {{500, 500, 500, 500, 500, 500}};
   b.cls = {6};
@@ -300,18 +341,32 @@ static void plan_invariants_random() {
     b.cls.push_back(0);
   }
   const gfft::PPlan p = plan_of(b, 8, 8);
-  CHECK(gfft::check_plan(p) == 0 && p.teams[0].k >= 2 && p.teams[0].k <= 8);
+  CHECK(gfft::check_plan(p) == 0 && p.teams[0].nparts == 2 && p.teams[0].k > 8 && p.teams[0].k <= 16);
+  CHECK(p.parts[p.teams[0].part0].dom != p.parts[p.teams[0].part0 + 1].dom);
   CHECK(p.solo.size() == 50 && p.team_ids.size() == 1);
+  // spanning switched off: one domain, at most its slots
+  const gfft::PPlan one = gfft::build_plan(
+      (uint32_t)b.tiles.size(), 6, kRoundTrip, [&](uint32_t t, int l) { return b.tiles[t][l]; },
+      [&](uint32_t t, int) { return b.cls[t]; },
+      [&](uint32_t t) {
+        double x = 0;
+        for (uint32_t n : b.tiles[t]) x += n;
+        return x;
+      },
+      8, 8, 2.0, 2, 2, 1);
+  CHECK(gfft::check_plan(one) == 0 && one.teams[0].nparts == 1 && one.teams[0].k == 8);
 }
 
 static void simulated_kernel_completes() {
   std::mt19937 rng(9);
   const int resident_counts[] = {1, 3, 8, 1000};
   const Spread spreads[] = {Spread::RoundRobin, Spread::AllInZero, Spread::Random, Spread::HalfOutside};
+  int spans = 0;
   for (int trial = 0; trial < 6; ++trial) {
     const Batch b = random_batch(rng, 6 + (int)(rng() % 20));
     const gfft::PPlan p = plan_of(b, 4, 6, 1 + trial % 3);
     CHECK(gfft::check_plan(p) == 0);
+    for (uint32_t t : p.team_ids) spans += p.teams[t].nparts > 1;
     for (Spread sp : spreads)
       for (int r : resident_counts) {
         Ctr ctr(gfft::counter_words(p));
@@ -327,6 +382,29 @@ static void simulated_kernel_completes() {
         }
       }
   }
+  CHECK(spans > 0);  // span teams were among the simulated plans
+}
+
+// One transform whose team spans two domains of four slots, every workgroup
+// resident and spread round robin, so both parts run shares of every level:
+// without the parts' releases a level's shares start before the other part's
+// stores are written back.
+static void span_team_levels_wait_for_every_part() {
+  Batch b;
+  b.tiles = {{64, 64, 64, 64, 64, 64}};
+  b.cls = {6};
+  const gfft::PPlan p = plan_of(b, 2, 4);
+  CHECK(gfft::check_plan(p) == 0 && p.teams[0].nparts == 2 && p.teams[0].k == 8);
+  Ctr ctr(gfft::counter_words(p));
+  for (auto& c : ctr) c.store(0);
+  for (int rep = 0; rep < 2; ++rep) {
+    const SimResult good = simulate(p, ctr, 1000, true, std::chrono::milliseconds(20000), 7 + rep,
+                                    Spread::RoundRobin);
+    CHECK(!good.failed && good.exactly_once && good.order_ok && good.domain_ok && good.reset);
+  }
+  const SimResult bad = simulate(p, ctr, 1000, true, std::chrono::milliseconds(20000), 9, Spread::RoundRobin,
+                                 true, false);
+  CHECK(!bad.order_ok);
 }
 
 static void no_stealing_with_absent_members_ends_by_the_bound() {
@@ -605,6 +683,7 @@ int main() {
   simulated_kernel_completes();
   no_stealing_with_absent_members_ends_by_the_bound();
   helpers_leave_orphaned_teams_alone();
+  span_team_levels_wait_for_every_part();
   wide_plan_invariants_random();
   simulated_wide_kernel_completes();
   wide_no_stealing_with_absent_workgroups_ends_by_the_bound();

@@ -212,7 +212,7 @@ def test_tuned_batch_matches_oracle():
         faster = "persistent" if chosen["tuned_ms"]["persistent"] <= chosen["tuned_ms"]["levels"] else "levels"
         assert chosen["scheduler"] == faster
         p = chosen["persist"]
-        assert 0 <= p["set"] <= 1 and p["team_factor"] in (0, 1, 2, 4)
+        assert 0 <= p["set"] <= 1 and p["team_factor"] in (0, 1, 2, 4) and b.span() in (1, 2)
         assert set(chosen["variants"]) == {d[2] for d in dims}
         bufs, host = inputs(torch, dims, 600)
         b.set_data(bufs)
@@ -419,6 +419,41 @@ def test_persistent_zero_wait_bound_drains():
         for (nx, ny, nz), buf, fp in zip(dims, work, host):
             got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
             assert O.rel_l2(got, O.r2c_p1(fp.astype(np.float64), nz)) <= 1e-6
+
+
+@pytest.mark.parametrize("dims", [[(192, 192, 192)], [(256, 256, 256), (160, 160, 160)] + [(32, 32, 32)] * 12])
+def test_span_teams_match_one_xcd_teams_bitwise(dims):
+    """A transform whose share of workgroups exceeds one XCD's slots runs as
+    a team spanning two XCDs (span 2, the default), each level handed over
+    through one L2 write-back per XCD part; with span 1 the same transform
+    runs as a team in one XCD.  The same tile code runs either way, so the
+    round trips are bitwise equal, and they match the FP64 oracle; three
+    repetitions (a part reading lines another XCD had not written back would
+    show as large errors)."""
+    import torch
+    for rep in range(3):
+        say(f"case span teams {dims[0]} +{len(dims) - 1}, rep {rep}")
+        out = {}
+        for span in (2, 1):
+            bufs, host = inputs(torch, dims, 1500 + rep)
+            with G.Batch(dims) as b:
+                b.set_persist(1, 2)
+                b.set_span(span)
+                assert b.span() == span
+                info = b.plan_info(G.ROUND_TRIP)
+                if len(dims) == 1:  # one team: more members than one XCD's slots only when it spans
+                    assert (info["members"] > info["slots"] // 8) == (span == 2), info
+                b.set_scales(c2r=[1.0 / (nx * ny * nz) for nx, ny, nz in dims])
+                b.set_data(bufs)
+                b.exec(G.ROUND_TRIP)
+                torch.cuda.synchronize()
+                assert b.status() == 0
+            out[span] = bufs
+            for (nx, ny, nz), buf, fp in zip(dims, bufs, host):
+                err = O.rel_l2(buf.cpu().numpy()[:, :, :nz], fp[:, :, :nz])
+                assert err <= 1e-6, ((nx, ny, nz), span, rep, err)
+        for x, y, d in zip(out[2], out[1], dims):
+            assert torch.equal(x, y), d
 
 
 def test_single_transform_teams_repeated_match_oracle():
