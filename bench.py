@@ -238,6 +238,7 @@ def batch_bench(args):
     # autotune the per-size kernel variants outside the timed region (the
     # normalised round trips keep the data at its magnitude)
     chosen = batch.tune(G.ROUND_TRIP, reps=3, stream=stream)
+    tuned_span = batch.span()
 
     def timed(fn):
         for _ in range(args.warmup):
@@ -297,6 +298,7 @@ def batch_bench(args):
            "plan": batch.plan_info(G.ROUND_TRIP),
            "tuned": {"scheduler": chosen["scheduler"], "tuned_eager_ms": chosen["tuned_ms"],
                      "persistent": {"set": chosen["persist"]["set"], "team_factor": chosen["persist"]["team_factor"],
+                                    "span": tuned_span,
                                     "variants": {str(k): v for k, v in chosen["persist"]["variants"].items()}},
                      "level_variants": {str(k): v for k, v in chosen["variants"].items()}},
            "schedulers": sched_ms}
