@@ -212,8 +212,10 @@ const int kModeAxis[3][gfft::batch::kMaxLevels] = {{0, 1, 3, 2}, {2, 3, 1, 0}, {
 const int kModeInv[3][gfft::batch::kMaxLevels] = {{0, 0, 0, 0}, {1, 1, 1, 1}, {0, 0, 0, 0, 1, 1, 1, 1}};
 const int kModeLevels[3] = {4, 4, 8};
 // Shares per team member and level in the persistent plans: a member runs
-// its own in turn, so a slow member's later ones can be taken by the others.
-constexpr uint32_t kSharesPerMember = 2;
+// its own in turn, so a slow member's later ones can be taken by the others
+// (and by helpers).  Three measured ~1 % faster than two on the bench mix
+// with helpers and span teams; four was slower (up to +15 % with spans).
+constexpr uint32_t kSharesPerMember = 3;
 // Global-level plans: about three shares per slot and level (the level's
 // tail is then short), and the transforms below a quarter of a slot share of work in
 // the solo queue, run whole while workgroups wait at level hand-overs.
