@@ -675,40 +675,53 @@ int time_exec(gfft_batch_s* b, int mode, int reps, hipStream_t s, hipEvent_t e0,
 }
 
 // Persistent scheduler: every variant set with each (team factor, span)
-// pair, team factor 0 being the global-level plan; a candidate must win by
-// 1 %; a candidate whose kernel does not launch is skipped.
+// pair, team factor 0 being the global-level plan.  Two passes over the
+// candidates, each candidate's time the faster of its two (the passes
+// interleave, so a slow moment of the device does not decide alone); a
+// candidate must win by 1 %; a set whose kernel does not launch is skipped.
 int tune_persistent(gfft_batch_s* b, int mode, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* best) {
   static const int kCandidates[][2] = {{0, 2}, {1, 2}, {2, 2}, {4, 2}, {1, 1}, {2, 1}};
-  int st = GFFT_OK, best_set = -1, best_factor = b->pteam, best_span = b->pspan;
-  float best_ms = 1e30f;
+  constexpr int kCand = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
+  float ms_of[gfft::batch::kSets][kCand];
+  bool runs[gfft::batch::kSets];
+  for (int set = 0; set < gfft::batch::kSets; ++set) {
+    runs[set] = true;
+    for (int i = 0; i < kCand; ++i) ms_of[set][i] = 1e30f;
+  }
+  int st = GFFT_OK;
   b->sched = GFFT_BATCH_PERSISTENT;
-  for (int set = 0; set < gfft::batch::kSets && st == GFFT_OK; ++set)
-    for (const auto& c : kCandidates) {
-      const int f = c[0];
-      b->pset = set;
-      b->pteam = f;
-      b->pspan = c[1];
-      if ((st = build_plans(b)) != GFFT_OK) break;
-      float ms = 0;
-      const int rt = time_exec(b, mode, reps, s, e0, e1, &ms);
-      if (rt == GFFT_ERR_HIP) {  // this set's kernel cannot run here: skip it
-        (void)hipGetLastError();
-        if (hipDeviceSynchronize() != hipSuccess) return GFFT_ERR_HIP;
-        break;
+  for (int pass = 0; pass < 2 && st == GFFT_OK; ++pass)
+    for (int set = 0; set < gfft::batch::kSets && st == GFFT_OK; ++set)
+      for (int i = 0; i < kCand && runs[set]; ++i) {
+        b->pset = set;
+        b->pteam = kCandidates[i][0];
+        b->pspan = kCandidates[i][1];
+        if ((st = build_plans(b)) != GFFT_OK) break;
+        float ms = 0;
+        const int rt = time_exec(b, mode, reps, s, e0, e1, &ms);
+        if (rt == GFFT_ERR_HIP) {  // this set's kernel cannot run here: skip it
+          (void)hipGetLastError();
+          if (hipDeviceSynchronize() != hipSuccess) return GFFT_ERR_HIP;
+          runs[set] = false;
+          break;
+        }
+        if ((st = rt) != GFFT_OK) break;
+        ms_of[set][i] = std::min(ms_of[set][i], ms);
       }
-      if ((st = rt) != GFFT_OK) break;
-      if (ms < 0.99f * best_ms) {
-        best_ms = ms;
-        best_set = set;
-        best_factor = f;
-        best_span = c[1];
-      }
-    }
   if (st != GFFT_OK) return st;
+  int best_set = -1, best_i = 0;
+  float best_ms = 1e30f;
+  for (int set = 0; set < gfft::batch::kSets; ++set)
+    for (int i = 0; i < kCand; ++i)
+      if (runs[set] && ms_of[set][i] < 0.99f * best_ms) {
+        best_ms = ms_of[set][i];
+        best_set = set;
+        best_i = i;
+      }
   if (best_set < 0) return GFFT_ERR_HIP;
   b->pset = best_set;
-  b->pteam = best_factor;
-  b->pspan = best_span;
+  b->pteam = kCandidates[best_i][0];
+  b->pspan = kCandidates[best_i][1];
   *best = best_ms;
   return build_plans(b);
 }
