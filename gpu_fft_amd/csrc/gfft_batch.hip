@@ -75,6 +75,41 @@ __global__ __launch_bounds__(kBT) void k_batch_plane(LevelArgs a) {
   }
 }
 
+// Fused z + y level of the 32 and 48 classes: one x-plane per wavefront
+// (four per workgroup), each wavefront striding over the level's planes on
+// its own and prefetching its next one (wave_planes).
+template <int C, int V>
+__global__ __launch_bounds__(kBT) void k_batch_plane_w(LevelArgs a) {
+  using PP = BPlaneW<C, V>;
+  constexpr uint32_t NW = kBT / 64;
+  __shared__ cx<float> smem[NW * (PP::PLANE + PP::SCRATCH)];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t k0 = blockIdx.x * NW + w, dk = gridDim.x * NW;
+  wave_planes<PP>(
+      [&](uint32_t i, const BXform** px, uint32_t* plane) {
+        const uint32_t k = k0 + i * dk;
+        if (k >= a.total) return false;
+        int lo = 0, hi = a.count;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (a.prefix[mid] <= k)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        *px = a.xf + lo;
+        *plane = k - a.prefix[lo];
+        return true;
+      },
+      a.inv != 0, a.tw, smem + w * (PP::PLANE + PP::SCRATCH));
+}
+
+// Planes per workgroup of a plane level's kernel.
+template <int C>
+constexpr uint32_t planes_per_wg() {
+  return C <= 1 ? kBT / 64 : 1u;
+}
+
 using LaunchFn = hipError_t (*)(dim3, const LevelArgs&, hipStream_t);
 
 template <int C, int V>
@@ -85,7 +120,10 @@ hipError_t launch_cv(dim3 grid, const LevelArgs& a, hipStream_t s) {
 
 template <int C, int V>
 hipError_t launch_plane_cv(dim3 grid, const LevelArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch_plane<C, V>), grid, dim3(kBT), 0, s, a);
+  if constexpr (C <= 1)
+    hipLaunchKernelGGL((k_batch_plane_w<C, V>), grid, dim3(kBT), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_batch_plane<C, V>), grid, dim3(kBT), 0, s, a);
   return hipGetLastError();
 }
 
@@ -96,16 +134,21 @@ struct VarInfo {
   LaunchFn launch;
   const void* plane_kernel;  // fused z + y level (plane classes), else null
   LaunchFn plane_launch;
+  uint32_t plane_tiles;      // planes per workgroup of the plane kernel
 };
 
 template <int C, int V>
 VarInfo var_info() {
   using F = BCfg<C, V>;
   VarInfo vi{make_sched<typename F::SC>(), F::P, F::W, reinterpret_cast<const void*>(&k_batch_level<C, V>),
-             &launch_cv<C, V>, nullptr, nullptr};
+             &launch_cv<C, V>, nullptr, nullptr, 1u};
   if constexpr (C < kPlaneClasses) {
-    vi.plane_kernel = reinterpret_cast<const void*>(&k_batch_plane<C, V>);
+    if constexpr (C <= 1)
+      vi.plane_kernel = reinterpret_cast<const void*>(&k_batch_plane_w<C, V>);
+    else
+      vi.plane_kernel = reinterpret_cast<const void*>(&k_batch_plane<C, V>);
     vi.plane_launch = &launch_plane_cv<C, V>;
+    vi.plane_tiles = planes_per_wg<C>();
   }
   return vi;
 }
@@ -316,8 +359,10 @@ gfft::batch::LevelArgs level_args(const gfft_batch_s* b, int c, int mode, int l)
 }
 
 uint32_t level_grid(const gfft_batch_s* b, int c, int axis, uint32_t total) {
-  const int occ = axis == gfft::batch::kAxisPlane ? b->occ_plane[c][b->var[c]] : b->occ[c][b->var[c]];
-  return std::min<uint32_t>(total, (uint32_t)(b->cus * occ));
+  const bool plane = axis == gfft::batch::kAxisPlane;
+  const int occ = plane ? b->occ_plane[c][b->var[c]] : b->occ[c][b->var[c]];
+  const uint32_t per = plane ? gfft::var(c, b->var[c]).plane_tiles : 1u;  // planes one workgroup runs at once
+  return std::min<uint32_t>((total + per - 1) / per, (uint32_t)(b->cus * occ));
 }
 
 // The plain variant: the class's transforms one after another through their

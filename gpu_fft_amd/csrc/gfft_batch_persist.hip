@@ -234,63 +234,20 @@ struct Seq {
 
   // x-planes through the plane pass, one wavefront per plane: wavefront w
   // takes elements w, w + 4, ... of the tile list (first, first + step, ...)
-  // and runs each in an LDS region of its own with wavefront-level barriers
-  // only, the next plane's loads in flight meanwhile, as in planes().
+  // (wave_planes in gfft_batch_tiles.h).
   __device__ static void planes_wave(const BXform& x, bool inv, uint32_t first, uint32_t step, uint32_t count,
                                      const cx<float>* stw, cx<float>* smem) {
     using PP = BPlaneW<F::CLS, F::VAR>;
-    constexpr int U = (PP::PLANE + 63) / 64;
-    constexpr bool kFull = U * 64 == PP::PLANE;
     const uint32_t w = threadIdx.x >> 6;
-    const int lane = (int)(threadIdx.x & 63u);
-    cx<float>* L = smem + w * (PP::PLANE + PP::SCRATCH);
-    cx<float>* base = reinterpret_cast<cx<float>*>(x.data);
-    cx<float> nxt[U];
-    if (w < count) {
-      const cx<float>* g = base + (size_t)(first + w * step) * PP::PLANE;
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = u * 64 + lane;
-        if (kFull || i < PP::PLANE) nxt[u] = ld_stream(g + i);
-      }
-    }
-    for (uint32_t k = w; k < count; k += kWaves) {
-      cx<float>* g = base + (size_t)(first + k * step) * PP::PLANE;
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = u * 64 + lane;
-        if (kFull || i < PP::PLANE) L[i] = nxt[u];
-      }
-      wave_sync();
-      if (k + kWaves < count) {
-        const cx<float>* gn = base + (size_t)(first + (k + kWaves) * step) * PP::PLANE;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int i = u * 64 + lane;
-          if (kFull || i < PP::PLANE) nxt[u] = ld_stream(gn + i);
-        }
-      }
-      if (!inv) {
-        PP::z_r2c(L, stw);
-        wave_sync();
-        // the forward scale belongs to this pass when x is length 1
-        const float sc = x.cls[2] < 0 ? x.scale_r2c : 1.0f;
-        PP::template y_pass<false>(L, L + PP::PLANE, stw, sc, sc != 1.0f);
-        wave_sync();
-        for (int i = lane; i < PP::PLANE; i += 64) st_stream(g + i, L[i]);
-      } else {
-        PP::template y_pass<true>(L, L + PP::PLANE, stw, 1.0f, false);
-        wave_sync();
-        PP::z_c2r(L, stw, x.scale_c2r, x.scale_c2r != 1.0f);
-        wave_sync();
-        constexpr int H = PP::NZ / 2;
-        for (int i = lane; i < PP::NY * H; i += 64) {
-          const int y = i / H, m = i - y * H;
-          st_stream(g + y * PP::K + m, L[y * PP::K + m]);
-        }
-      }
-      wave_sync();  // the stores' LDS reads are done before the next plane lands
-    }
+    const BXform* xp = &x;
+    wave_planes<PP>(
+        [&](uint32_t i, const BXform** px, uint32_t* plane) {
+          const uint32_t e = w + i * kWaves;
+          *px = xp;
+          *plane = first + e * step;
+          return e < count;
+        },
+        inv, stw, smem + w * (PP::PLANE + PP::SCRATCH));
   }
 
   template <bool INV>

@@ -247,6 +247,74 @@ __device__ __forceinline__ void plane_tile(const BXform& x, bool inv, uint32_t p
   __syncthreads();
 }
 
+// Planes through a one-wavefront plane pass PP (PP::NT == 64, wavefront-level
+// barriers): the calling wavefront runs the planes next(0), next(1), ...
+// (next(i, &x, &plane) returns false past its last) in its own LDS region L
+// of PP::PLANE + PP::SCRATCH entries, loading each next plane into registers
+// before the current one's stages, so its loads are in flight meanwhile.
+// R2C runs z then y and writes the half spectra back, C2R y then z and the
+// Nz reals of every row; the scales are each plane's transform's (the
+// forward one belongs to this pass when x is length 1).  Planes must be
+// disjoint (loading the next before storing the current is then safe in
+// place).
+template <typename PP, typename Next>
+__device__ __forceinline__ void wave_planes(Next next, bool inv, const cx<float>* tw, cx<float>* L) {
+  static_assert(PP::NT == 64, "one plane per wavefront");
+  constexpr int U = (PP::PLANE + 63) / 64;
+  constexpr bool kFull = U * 64 == PP::PLANE;
+  const int lane = (int)(threadIdx.x & 63u);
+  cx<float> nxt[U];
+  const BXform* nx = nullptr;
+  uint32_t np = 0;
+  bool more = next(0u, &nx, &np);
+  if (more) {
+    const cx<float>* g = reinterpret_cast<const cx<float>*>(nx->data) + (size_t)np * PP::PLANE;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = u * 64 + lane;
+      if (kFull || i < PP::PLANE) nxt[u] = ld_stream(g + i);
+    }
+  }
+  for (uint32_t k = 0; more; ++k) {
+    const BXform* x = nx;
+    cx<float>* g = reinterpret_cast<cx<float>*>(x->data) + (size_t)np * PP::PLANE;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = u * 64 + lane;
+      if (kFull || i < PP::PLANE) L[i] = nxt[u];
+    }
+    wave_sync();
+    more = next(k + 1, &nx, &np);
+    if (more) {
+      const cx<float>* gn = reinterpret_cast<const cx<float>*>(nx->data) + (size_t)np * PP::PLANE;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = u * 64 + lane;
+        if (kFull || i < PP::PLANE) nxt[u] = ld_stream(gn + i);
+      }
+    }
+    if (!inv) {
+      PP::z_r2c(L, tw);
+      wave_sync();
+      const float sc = x->cls[2] < 0 ? x->scale_r2c : 1.0f;
+      PP::template y_pass<false>(L, L + PP::PLANE, tw, sc, sc != 1.0f);
+      wave_sync();
+      for (int i = lane; i < PP::PLANE; i += 64) st_stream(g + i, L[i]);
+    } else {
+      PP::template y_pass<true>(L, L + PP::PLANE, tw, 1.0f, false);
+      wave_sync();
+      PP::z_c2r(L, tw, x->scale_c2r, x->scale_c2r != 1.0f);
+      wave_sync();
+      constexpr int H = PP::NZ / 2;
+      for (int i = lane; i < PP::NY * H; i += 64) {
+        const int y = i / H, m = i - y * H;
+        st_stream(g + y * PP::K + m, L[y * PP::K + m]);
+      }
+    }
+    wave_sync();  // the stores' LDS reads are done before the next plane lands
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Persistent execution (gfft_batch_persist.hip): one launch runs the plan of
 // a mode (gfft_batch_sched.h).  A variant set fixes one compiled variant per
