@@ -28,24 +28,25 @@
 // released: one workgroup of the part (the first to claim the part's release
 // word) writes its XCD's L2 back (agent-scope release fence, then the flag),
 // and the acquire after the wait drops stale lines of the other parts' L2s.
-// Otherwise no L2 write-back happens inside the launch.  (A workgroup-scope L1
-// invalidate, buffer_inv sc0, is not enough outside threadgroup-split mode:
-// with it, lines a member had read in an earlier level were now and then
-// served stale from its L1.)  While
-// waiting, after steal_ticks, a member claims and runs any share still free
-// (its owner has not started), so a team never depends on all of its members
-// being resident.  Every wait is bounded: it gives up when the launch's
-// wall-clock bound passes or when the host sets the batch's abort word
-// (pinned host memory, system-scope loads), and marks the launch failed; a
-// failed launch's workgroups stop computing and leave.  A workgroup that
-// finds the solo queue empty helps the teams of its own domain (same L2):
-// it claims free shares of each team's current level, runs them and marks
-// them done as a member would, after an agent-scope acquire (the level
-// before is complete in the L2), and leaves when those teams are complete.  The last workgroup to
-// leave runs the teams of which no member arrived (whole, alone), moves the
-// failure bits into the sticky status word and zeroes the other counter
-// words, so the next launch (or graph replay) starts from zero without a
-// memset.
+// Otherwise no L2 write-back happens inside the launch.  (A workgroup-scope
+// L1 invalidate, buffer_inv sc0, is not enough outside threadgroup-split
+// mode: with it, lines a member had read in an earlier level were now and
+// then served stale from its L1.)
+//
+// While waiting, after steal_ticks, a member claims and runs any share still
+// free (its owner has not started), so a team never depends on all of its
+// members being resident.  Every wait is bounded: it gives up when the
+// launch's wall-clock bound passes or when the host sets the batch's abort
+// word (pinned host memory, system-scope loads), and marks the launch
+// failed; a failed launch's workgroups stop computing and leave.  A
+// workgroup that finds the solo queue empty helps the teams of its own
+// domain (same L2): it claims free shares of each team's current level, runs
+// them and marks them done as a member would, after an agent-scope acquire
+// (the level before is complete in the L2), and leaves when those teams are
+// complete.  The last workgroup to leave runs the teams of which no member
+// arrived (whole, alone), moves the failure bits into the sticky status word
+// and zeroes the other counter words, so the next launch (or graph replay)
+// starts from zero without a memset.
 //
 // With a global-level plan (team factor 0, WPlan in gfft_batch_sched.h) every
 // workgroup takes shares of every transform's level g in level g of the
