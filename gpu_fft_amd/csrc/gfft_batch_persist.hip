@@ -102,11 +102,12 @@ struct Seq {
   using COLS = ColsPass<float, INV, SC, P, W, true>;
   static constexpr int kTw = twiddle_count<SC>();
   static constexpr bool kPlane = F::CLS < kPersistPlaneClasses;  // z + y fused levels
-  // plane loops: one wavefront per plane for 32 and 48 (a 256-thread plane
-  // of those leaves most threads idle in its stages), the workgroup on one
-  // plane, pipelined, for 64; 96 and 128 unpipelined (the next plane's 19 or
-  // 33 entries per thread pushed their functions into accumulator registers
-  // and the kernel to one wave per SIMD)
+  // plane loops (in functions of their own, class_planes, so that they do
+  // not add to the rows' and columns' registers): one wavefront per plane
+  // for 32 and 48 (a 256-thread plane of those leaves most threads idle in
+  // its stages), the workgroup on one plane, pipelined, for 64; 96 and 128
+  // unpipelined (the next plane's 19 or 33 entries per thread spill into
+  // accumulator registers, which takes the kernel to one wave per SIMD)
   static constexpr bool kPlaneWave = kPlane && F::CLS <= 1;
   static constexpr bool kPlanePipe = kPlane && F::CLS == 2;
   static constexpr int kWaves = kBT / 64;
@@ -290,16 +291,7 @@ struct Seq {
     for (int i = threadIdx.x; i < SC::twoff(SC::S); i += kBT) stw[i] = tw[i];
     __syncthreads();  // the table is complete (and the previous call's LDS reads are done)
     if (count > 0) {
-      if (axis == kAxisPlane) {
-        if constexpr (kPlaneWave) {
-          planes_wave(x, inv != 0, first, step, count, stw, smem);
-        } else if constexpr (kPlanePipe) {
-          planes(x, inv != 0, first, step, count, stw, smem);
-        } else if constexpr (kPlane) {
-          for (uint32_t k = 0; k < count; ++k)
-            plane_tile<BPlane<F::CLS, F::VAR>>(x, inv != 0, first + k * step, stw, smem);
-        }
-      } else if (axis == 0) {
+      if (axis == 0) {
         if (inv)
           rows_c2r(x, first, step, count, stw, smem);
         else
@@ -308,6 +300,25 @@ struct Seq {
         cols<true>(x, axis, first, step, count, stw, smem);
       } else {
         cols<false>(x, axis, first, step, count, stw, smem);
+      }
+    }
+    __syncthreads();  // every LDS access of this call is done before the next call's writes
+  }
+
+  // the fused plane level's tiles (plane classes only)
+  __device__ __forceinline__ static void run_planes(const BXform& x, int inv, uint32_t first, uint32_t step,
+                                                    uint32_t count, const cx<float>* tw, cx<float>* smem) {
+    cx<float>* stw = smem + base_entries();
+    for (int i = threadIdx.x; i < SC::twoff(SC::S); i += kBT) stw[i] = tw[i];
+    __syncthreads();  // the table is complete (and the previous call's LDS reads are done)
+    if (count > 0) {
+      if constexpr (kPlaneWave) {
+        planes_wave(x, inv != 0, first, step, count, stw, smem);
+      } else if constexpr (kPlanePipe) {
+        planes(x, inv != 0, first, step, count, stw, smem);
+      } else if constexpr (kPlane) {
+        for (uint32_t k = 0; k < count; ++k)
+          plane_tile<BPlane<F::CLS, F::VAR>>(x, inv != 0, first + k * step, stw, smem);
       }
     }
     __syncthreads();  // every LDS access of this call is done before the next call's writes
@@ -324,9 +335,28 @@ __device__ __attribute__((noinline)) void class_tiles(const BXform& x, int axis,
   Seq<SCfg<S, C>>::run(x, axis, inv, first, step, count, tw, smem);
 }
 
+// A plane class's fused plane levels, in a function of its own (see
+// class_tiles).
+template <int S, int C>
+__device__ __attribute__((noinline)) void class_planes(const BXform& x, int inv, uint32_t first, uint32_t step,
+                                                       uint32_t count, const cx<float>* tw, cx<float>* smem) {
+  Seq<SCfg<S, C>>::run_planes(x, inv, first, step, count, tw, smem);
+}
+
 template <int S>
 __device__ __forceinline__ void run_set_tiles(const BXform& x, const PSeg& sg, uint32_t first, uint32_t step,
                                               uint32_t count, const PersistArgs& a, cx<float>* smem) {
+  static_assert(kPersistPlaneClasses == 5, "plane classes 0 .. 4");
+  if (sg.axis == kAxisPlane) {
+    switch (sg.cls) {
+      case 0: class_planes<S, 0>(x, sg.inv, first, step, count, a.tw[0], smem); break;
+      case 1: class_planes<S, 1>(x, sg.inv, first, step, count, a.tw[1], smem); break;
+      case 2: class_planes<S, 2>(x, sg.inv, first, step, count, a.tw[2], smem); break;
+      case 3: class_planes<S, 3>(x, sg.inv, first, step, count, a.tw[3], smem); break;
+      default: class_planes<S, 4>(x, sg.inv, first, step, count, a.tw[4], smem); break;
+    }
+    return;
+  }
   switch (sg.cls) {
     case 0: class_tiles<S, 0>(x, sg.axis, sg.inv, first, step, count, a.tw[0], smem); break;
     case 1: class_tiles<S, 1>(x, sg.axis, sg.inv, first, step, count, a.tw[1], smem); break;
