@@ -189,21 +189,53 @@ struct PlanePass {
     }
   }
 
-  // y C2C of every kz column of the LDS plane, in place
+  // y C2C of every kz column of the LDS plane, in place.  The DC and Nyquist
+  // columns (kz = 0 and NZ / 2) are transformed together, as one complex
+  // column, so NZ / 2 columns take the rounds of CW instead of NZ / 2 + 1:
+  //   R2C: both columns are real after the z pass (its two-for-one split
+  //        leaves their imaginary parts exactly zero), so column 0 takes
+  //        c = x0 + i xN; after the transform the two spectra are the
+  //        Hermitian and anti-Hermitian parts of C: X0[k] = (C[k] +
+  //        conj(C[-k])) / 2, XN[k] = (C[k] - conj(C[-k])) / 2i;
+  //   C2R: the z pass reads only the real parts of the two columns (numpy's
+  //        irfft convention), which are the inverse transforms of their
+  //        Hermitian parts h0, hN; column 0 takes h0 + i hN, whose inverse
+  //        transform has x0 as its real part and xN as its imaginary part.
   template <bool INV>
   __device__ __forceinline__ static void y_pass(C* plane, C* scratch, const C* __restrict__ tw, T sc, bool scaled) {
     using E = RegEngine<T, INV, SCY, PY, BMY>;
+    constexpr int H = NZ / 2;  // the Nyquist column; columns 1 .. H - 1 and the packed column 0
+    const T half = (T)0.5;
+    if constexpr (INV) {
+      // Hermitian parts of columns 0 and H, packed into column 0 (pairs k, -k)
+      for (int p = lid(); p <= NY / 2; p += NT) {
+        const int k1 = p, k2 = (NY - p) % NY;
+        const C a1 = plane[k1 * K], a2 = plane[k2 * K], b1 = plane[k1 * K + H], b2 = plane[k2 * K + H];
+        const C h0 = mk<T>((a1.x + a2.x) * half, (a1.y - a2.y) * half);  // h0[k2] = conj(h0[k1])
+        const C hn = mk<T>((b1.x + b2.x) * half, (b1.y - b2.y) * half);
+        plane[k1 * K] = mk<T>(h0.x - hn.y, h0.y + hn.x);
+        if (k2 != k1) plane[k2 * K] = mk<T>(h0.x + hn.y, hn.x - h0.y);
+      }
+      stage_barrier<BMY>();
+    }
     const int c = lid() % CW, t = lid() / CW;
-    for (int k0 = 0; k0 < K; k0 += CW) {
+    for (int k0 = 0; k0 < H; k0 += CW) {
       const int k = k0 + c;
-      const bool valid = k < K;
+      const bool valid = k < H;
       C v[E::PV];
 #pragma unroll
       for (int q = 0; q < E::NB0; ++q) {
         if (!E::live0(t, q)) continue;
 #pragma unroll
-        for (int r = 0; r < E::R0; ++r)
-          v[q * E::R0 + r] = valid ? plane[E::in_index(t, q, r) * K + k] : mk<T>(0, 0);
+        for (int r = 0; r < E::R0; ++r) {
+          const int n = E::in_index(t, q, r);
+          if (!valid)
+            v[q * E::R0 + r] = mk<T>(0, 0);
+          else if (!INV && k == 0)
+            v[q * E::R0 + r] = mk<T>(plane[n * K].x, plane[n * K + H].x);
+          else
+            v[q * E::R0 + r] = plane[n * K + k];
+        }
       }
       StrideLds<T> L{valid ? plane + k : scratch, valid ? K : 1};
       E::run(v, t, L, tw);
@@ -214,10 +246,36 @@ struct PlanePass {
           if (!E::liveL(t, q)) continue;
 #pragma unroll
           for (int r = 0; r < E::RL; ++r) {
+            const int n = E::out_index(t, q, r);
             C z = v[q * E::RL + r];
+            if (k == 0) {
+              if (INV) {  // x0 and xN, each real
+                plane[n * K] = mk<T>(z.x, (T)0);
+                plane[n * K + H] = mk<T>(z.y, (T)0);
+              } else {  // C, split below
+                plane[n * K] = z;
+              }
+              continue;
+            }
             if (scaled) z = mk<T>(z.x * sc, z.y * sc);
-            plane[E::out_index(t, q, r) * K + k] = z;
+            plane[n * K + k] = z;
           }
+        }
+      }
+    }
+    if constexpr (!INV) {
+      stage_barrier<BMY>();  // the packed column's transform is complete
+      const T f = scaled ? sc * half : half;
+      for (int p = lid(); p <= NY / 2; p += NT) {
+        const int k1 = p, k2 = (NY - p) % NY;
+        const C c1 = plane[k1 * K], c2 = plane[k2 * K];
+        const C x0 = mk<T>((c1.x + c2.x) * f, (c1.y - c2.y) * f);  // X0[k2] = conj(X0[k1])
+        const C xn = mk<T>((c1.y + c2.y) * f, (c2.x - c1.x) * f);  // XN[k2] = conj(XN[k1])
+        plane[k1 * K] = x0;
+        plane[k1 * K + H] = xn;
+        if (k2 != k1) {
+          plane[k2 * K] = mk<T>(x0.x, -x0.y);
+          plane[k2 * K + H] = mk<T>(xn.x, -xn.y);
         }
       }
     }
