@@ -112,19 +112,23 @@ constexpr uint32_t planes_per_wg() {
 
 using LaunchFn = hipError_t (*)(dim3, const LevelArgs&, hipStream_t);
 
+// Launches report their own status (hipLaunchKernel's return), not the
+// thread's last error, which an earlier tolerated failure may have left.
 template <int C, int V>
 hipError_t launch_cv(dim3 grid, const LevelArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch_level<C, V>), grid, dim3(kBT), 0, s, a);
-  return hipGetLastError();
+  void* args[] = {const_cast<LevelArgs*>(&a)};
+  return hipLaunchKernel(reinterpret_cast<const void*>(&k_batch_level<C, V>), grid, dim3(kBT), args, 0, s);
 }
 
 template <int C, int V>
 hipError_t launch_plane_cv(dim3 grid, const LevelArgs& a, hipStream_t s) {
+  void* args[] = {const_cast<LevelArgs*>(&a)};
+  const void* fn = nullptr;
   if constexpr (C <= 1)
-    hipLaunchKernelGGL((k_batch_plane_w<C, V>), grid, dim3(kBT), 0, s, a);
+    fn = reinterpret_cast<const void*>(&k_batch_plane_w<C, V>);
   else
-    hipLaunchKernelGGL((k_batch_plane<C, V>), grid, dim3(kBT), 0, s, a);
-  return hipGetLastError();
+    fn = reinterpret_cast<const void*>(&k_batch_plane<C, V>);
+  return hipLaunchKernel(fn, grid, dim3(kBT), args, 0, s);
 }
 
 struct VarInfo {
@@ -264,6 +268,9 @@ constexpr uint32_t kSharesPerMember = 3;
 // the solo queue, run whole while workgroups wait at level hand-overs.
 constexpr uint32_t kWideSharesPerSlot = 3;
 constexpr double kWideSoloFactor = 0.25;
+// plane tiles' cost per value moved relative to row and column tiles (bench
+// mix, global-level plan: weight 1 0.737 ms, 1.4 0.74, 1.8 0.71, 2.5 0.72)
+constexpr double kWidePlaneWeight = 1.8;
 
 // L2 domains (XCDs) of the current device; 1 where the count is unknown or
 // above what a plan holds (the kernel then gives team roles only to the
@@ -447,8 +454,10 @@ int build_plans(gfft_batch_s* b) {
   b->plds = persist_lds_bytes(b->pset, present);
   int occ = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, persist_kernel(b->pset), kBT, b->plds) != hipSuccess ||
-      occ < 1)
+      occ < 1) {
+    (void)hipGetLastError();  // tolerated: one workgroup per CU
     occ = 1;
+  }
   b->pocc = occ;
   const uint32_t domains = (uint32_t)b->xccs;
   const uint32_t per_dom = std::max(1u, (uint32_t)std::max(1, b->cus * occ) / domains);
@@ -463,13 +472,16 @@ int build_plans(gfft_batch_s* b) {
     };
     gfft_batch_s::DevPlan& dp = b->plan[m];
     if (b->pteam == 0) {
-      // relative cost of one tile: the complex values it moves
+      // relative cost of one tile: the complex values it moves, a fused
+      // plane's weighted by kWidePlaneWeight (its two stages and its load
+      // latency cost more per value than a row or column tile: per-workgroup
+      // stamps showed the slots heavy in planes ending each level's tail)
       auto cost = [&](uint32_t t, int l) {
         const BXform& x = b->hx[t];
         const int axis = kModeAxis[m][l];
         const int c = level_cls(x, axis, kPersistPlaneClasses);
         if (c < 0) return 0.0;
-        if (axis == kAxisPlane) return (double)x.ny * x.K;
+        if (axis == kAxisPlane) return kWidePlaneWeight * x.ny * x.K;
         if (axis == 0) return 2.0 * w[c] * x.K;
         return (double)w[c] * (axis == 1 ? x.ny : x.nrows / x.ny);
       };
@@ -909,14 +921,18 @@ int gfft_batch_create(gfft_batch* out, int count, const int64_t* dims, int prec)
         st = GFFT_ERR_HIP;
       int occ = 0;
       if (st == GFFT_OK &&
-          (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, var(c, v).kernel, kBT, 0) != hipSuccess || occ < 1))
+          (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, var(c, v).kernel, kBT, 0) != hipSuccess || occ < 1)) {
+        (void)hipGetLastError();  // tolerated: one workgroup per CU
         occ = 1;
+      }
       b->occ[c][v] = occ;
       occ = 0;
       if (st == GFFT_OK && var(c, v).plane_kernel &&
           (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, var(c, v).plane_kernel, kBT, 0) != hipSuccess ||
-           occ < 1))
+           occ < 1)) {
+        (void)hipGetLastError();
         occ = 1;
+      }
       b->occ_plane[c][v] = occ;
     }
   }

@@ -1091,8 +1091,9 @@ hipError_t static_counters(uint32_t** out, hipStream_t s) {
     if (!p) return hipErrorInvalidSymbol;
     // first use on this device: clear it with a kernel (stream-ordered, and a
     // kernel node if the caller captures the stream)
-    hipLaunchKernelGGL(k_fill_zero, dim3(64), dim3(256), 0, s, p, kStaticCounterWords);
-    e = hipGetLastError();
+    size_t n = kStaticCounterWords;
+    void* args[] = {&p, &n};
+    e = hipLaunchKernel(reinterpret_cast<const void*>(&k_fill_zero), dim3(64), dim3(256), args, 0, s);
     if (e != hipSuccess) return e;
     it = ptr.emplace(dev, p).first;
   }
@@ -1100,12 +1101,11 @@ hipError_t static_counters(uint32_t** out, hipStream_t s) {
   return hipSuccess;
 }
 
+// The launch's own status (not the thread's last error, which an earlier
+// tolerated failure elsewhere may have left).
 hipError_t launch_persist(int set, uint32_t grid, size_t lds, const PersistArgs& a, hipStream_t s) {
-  if (set == 0)
-    hipLaunchKernelGGL((k_batch_persist<0>), dim3(grid), dim3(kBT), lds, s, a);
-  else
-    hipLaunchKernelGGL((k_batch_persist<1>), dim3(grid), dim3(kBT), lds, s, a);
-  return hipGetLastError();
+  void* args[] = {const_cast<PersistArgs*>(&a)};
+  return hipLaunchKernel(persist_kernel(set), dim3(grid), dim3(kBT), args, lds, s);
 }
 
 }  // namespace batch
