@@ -240,12 +240,14 @@ hipError_t gen_rows(int kind, void* data, RowGeom g, const RtSchedHost& h, const
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gen_rows_r2c<T>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
+    clear_launch_error();
     hipLaunchKernelGGL(k_gen_rows_r2c<T>, dim3((uint32_t)blocks), dim3(kGenThreads), lds, s, static_cast<T*>(data),
                        g, sc, static_cast<const cx<T>*>(tw), G);
   } else {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gen_rows_c2r<T>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
+    clear_launch_error();
     hipLaunchKernelGGL(k_gen_rows_c2r<T>, dim3((uint32_t)blocks), dim3(kGenThreads), lds, s, static_cast<T*>(data),
                        g, sc, static_cast<const cx<T>*>(tw), G);
   }
@@ -270,6 +272,7 @@ hipError_t gen_cols(int kind, const void* in, void* out, ColGeom g, const RtSche
                                     : reinterpret_cast<const void*>(&k_gen_cols<T, false>);
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
+  clear_launch_error();
   if (kind == COLS_INV)
     hipLaunchKernelGGL((k_gen_cols<T, true>), dim3((uint32_t)blocks), dim3(kGenThreads), lds, s,
                        static_cast<const cx<T>*>(in), static_cast<cx<T>*>(out), g, sc, static_cast<const cx<T>*>(tw));

@@ -524,12 +524,18 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, RowsC2R<T, SC, P, G, tr
 
 int device_cu_count();
 
-// Host launchers with the registry's signatures.
+// Host launchers with the registry's signatures.  Each clears the thread's
+// last HIP error before its launch, so the hipGetLastError() read after it
+// reports that launch only, not a failure an earlier call left behind (the
+// occupancy queries tolerate theirs, and so may the caller's own code).
+inline void clear_launch_error() { (void)hipGetLastError(); }
+
 template <typename T, typename SC, int P, int G>
 hipError_t launch_rows_r2c(void* data, RowGeom g, const void* tw, hipStream_t s) {
   const int64_t pairs = (g.nrows + 1) / 2;
   const int64_t blocks = (pairs + G - 1) / G;
   if (blocks <= 0) return hipSuccess;
+  clear_launch_error();
   hipLaunchKernelGGL((k_rows_r2c<T, SC, P, G>), dim3((uint32_t)blocks), dim3(RowsR2C<T, SC, P, G>::NT), 0, s,
                      static_cast<T*>(data), g, static_cast<const cx<T>*>(tw));
   return hipGetLastError();
@@ -540,6 +546,7 @@ hipError_t launch_rows_c2r(void* data, RowGeom g, const void* tw, hipStream_t s)
   const int64_t pairs = (g.nrows + 1) / 2;
   const int64_t blocks = (pairs + G - 1) / G;
   if (blocks <= 0) return hipSuccess;
+  clear_launch_error();
   hipLaunchKernelGGL((k_rows_c2r<T, SC, P, G>), dim3((uint32_t)blocks), dim3(RowsC2R<T, SC, P, G>::NT), 0, s,
                      static_cast<T*>(data), g, static_cast<const cx<T>*>(tw));
   return hipGetLastError();
@@ -549,6 +556,7 @@ template <typename T, bool INV, typename SC, int P, int CW, int ORD = 0>
 hipError_t launch_cols(const void* in, void* out, ColGeom g, const void* tw, hipStream_t s) {
   const int64_t blocks = ((int64_t)g.ncols + CW - 1) / CW;
   if (blocks <= 0) return hipSuccess;
+  clear_launch_error();
   hipLaunchKernelGGL((k_cols<T, INV, SC, P, CW, ORD>), dim3((uint32_t)blocks), dim3(ColsPass<T, INV, SC, P, CW>::NT),
                      0, s, static_cast<const cx<T>*>(in), static_cast<cx<T>*>(out), g, static_cast<const cx<T>*>(tw));
   return hipGetLastError();
@@ -566,6 +574,7 @@ hipError_t launch_cols_pipe(const void* in, void* out, ColGeom g, const void* tw
   const int64_t ntiles = ((int64_t)g.ncols + CW - 1) / CW;
   if (ntiles <= 0) return hipSuccess;
   const dim3 grid((uint32_t)persistent_blocks(ntiles, OCC)), block(ColsPipe<T, INV, SC, P, CW>::NT);
+  clear_launch_error();
   if (FL && g.ncols % CW == 0 && g.in.nsub >= SC::N && g.out.nsub >= SC::N)
     hipLaunchKernelGGL((k_cols_pipe<T, INV, SC, P, CW, ORD, RA, true>), grid, block, 0, s,
                        static_cast<const cx<T>*>(in), static_cast<cx<T>*>(out), g, static_cast<const cx<T>*>(tw),
@@ -581,6 +590,7 @@ template <typename T, typename SC, int P, int G, int OCC, int ORD>
 hipError_t launch_rows_r2c_pipe(void* data, RowGeom g, const void* tw, hipStream_t s) {
   const int64_t ntiles = ((g.nrows + 1) / 2 + G - 1) / G;
   if (ntiles <= 0) return hipSuccess;
+  clear_launch_error();
   hipLaunchKernelGGL((k_rows_r2c_pipe<T, SC, P, G, ORD>), dim3((uint32_t)persistent_blocks(ntiles, OCC)),
                      dim3(RowsR2C<T, SC, P, G, true>::NT), 0, s, static_cast<T*>(data), g,
                      static_cast<const cx<T>*>(tw), (uint32_t)ntiles);
@@ -591,6 +601,7 @@ template <typename T, typename SC, int P, int G, int OCC, int ORD>
 hipError_t launch_rows_c2r_pipe(void* data, RowGeom g, const void* tw, hipStream_t s) {
   const int64_t ntiles = ((g.nrows + 1) / 2 + G - 1) / G;
   if (ntiles <= 0) return hipSuccess;
+  clear_launch_error();
   hipLaunchKernelGGL((k_rows_c2r_pipe<T, SC, P, G, ORD>), dim3((uint32_t)persistent_blocks(ntiles, OCC)),
                      dim3(RowsC2R<T, SC, P, G, true>::NT), 0, s, static_cast<T*>(data), g,
                      static_cast<const cx<T>*>(tw), (uint32_t)ntiles);

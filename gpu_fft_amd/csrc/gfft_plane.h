@@ -324,6 +324,7 @@ hipError_t launch_plane(void* data, int64_t nplanes, double scale, const void* t
                         hipStream_t s) {
   using K_ = PlanePass<T, SCZ, PZ, SCY, PY, CW>;
   if (nplanes <= 0) return hipSuccess;
+  clear_launch_error();
   if (INV)
     hipLaunchKernelGGL((k_plane_c2r<T, SCZ, PZ, SCY, PY, CW>), dim3((uint32_t)nplanes), dim3(K_::NT), 0, s,
                        static_cast<T*>(data), static_cast<const cx<T>*>(twz), static_cast<const cx<T>*>(twy),
