@@ -331,7 +331,7 @@ def pct(xs, q):
     return xs[k]
 
 
-def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: float = 64.0, nstreams: int = 4,
+def serve_bench(n_requests: int, staged: bool, load: float = 0.8, budget_gb: float = 64.0, nstreams: int = 3,
                 seed: int = 0, small_lane: int = -1, own_queues: bool = True, stream_depth: int = -1,
                 poll_us: int = -1) -> dict:
     """Poisson-arrival serving run of the config-5 mix.
@@ -657,7 +657,7 @@ def main():
     ap.add_argument("--serve-poll-us", type=int, default=-1, metavar="US",
                     help="server waiters' sleep between completion queries (0 = blocking, -1 = default 20)")
     ap.add_argument("--serve-device-only", action="store_true", help="serving: device requests only")
-    ap.add_argument("--serve-streams", type=int, default=4, help="server streams")
+    ap.add_argument("--serve-streams", type=int, default=3, help="server streams")
     ap.add_argument("--accuracy", action="store_true",
                     help="relative L2 error vs the FP64 oracle at every config grid (no timing)")
     ap.add_argument("--no-extras", action="store_true",

@@ -112,9 +112,14 @@ def _ptr(x) -> int:
 
 
 class Server:
-    """A serving runtime on the current HIP device."""
+    """A serving runtime on the current HIP device.
 
-    def __init__(self, budget_bytes: int, nstreams: int = 4, staging: bool = False, own_queues: bool = True):
+    nstreams = 3 by default: two streams for large requests and the
+    small-request lane.  On the mixed-size Poisson workload at 80 % load it
+    gave a lower p99 latency than 4 streams (6.1-6.2 ms against 6.4-6.5)
+    at 1 % less capacity; 2 streams lost 14 % of capacity."""
+
+    def __init__(self, budget_bytes: int, nstreams: int = 3, staging: bool = False, own_queues: bool = True):
         L = _bind(lib())
         h = ctypes.c_void_p()
         flags = (STAGING if staging else 0) | (0 if own_queues else SHARED_QUEUES)

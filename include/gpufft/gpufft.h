@@ -431,7 +431,10 @@ int gfft_request_bytes(int64_t nx, int64_t ny, int64_t nz, int prec, size_t* byt
  * of every CU, which gives it a hardware queue of its own (the streams then
  * synchronise with the legacy null stream, like any blocking stream); with
  * GFFT_SERVER_SHARED_QUEUES they are plain non-blocking streams, which the
- * runtime may map several to one hardware queue. */
+ * runtime may map several to one hardware queue.  nstreams = 3 (two streams
+ * for large requests and the small-request lane) measured the lowest p99
+ * latency on a mixed-size Poisson load at 80 % of capacity; more streams
+ * share the GPU among more large requests and stretch each one's service. */
 int gfft_server_create(gfft_server* srv, size_t budget_bytes, int nstreams, int flags);
 
 /* Reserve the last stream for requests of at most max_bytes (array bytes):
