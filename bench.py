@@ -756,7 +756,8 @@ def main():
         out["batch"]["launches"] = b["config"]["launches"]
         out["batch"]["workload"] = b["config"]["workload"]
         log("serving: device-resident requests")
-        out["serving"] = serve_bench(5000, staged=False, load=0.8)
+        # 10000 requests: the p99 then rests on the slowest 100 of them
+        out["serving"] = serve_bench(10000, staged=False, load=0.8)
         log("serving: host-staged requests")
         out["serving_host_staged"] = serve_bench(300, staged=True, load=0.8)
     print(json.dumps(out), flush=True)
