@@ -271,6 +271,7 @@ def batch_bench(args):
         print(f"batch: {name} scheduler, graph path", file=sys.stderr, flush=True)
         batch.set_scheduler(sch)
         sched_ms[name] = {"graph_ms": round(timed(lambda: batch.exec_graph(G.ROUND_TRIP, stream)), 5),
+                          "eager_ms": round(timed(lambda: batch.exec(G.ROUND_TRIP, stream)), 5),
                           "launches": batch.tiles(G.ROUND_TRIP)[1]}
     # the persistent kernel's global-level plan (team factor 0: every
     # workgroup in every transform's passes), whatever the tuner kept
@@ -278,6 +279,7 @@ def batch_bench(args):
     batch.set_scheduler(G.Batch.PERSISTENT)
     batch.set_persist(chosen["persist"]["set"], 0)
     sched_ms["persistent_global_levels"] = {"graph_ms": round(timed(lambda: batch.exec_graph(G.ROUND_TRIP, stream)), 5),
+                                            "eager_ms": round(timed(lambda: batch.exec(G.ROUND_TRIP, stream)), 5),
                                             "launches": batch.tiles(G.ROUND_TRIP)[1]}
     batch.set_persist(chosen["persist"]["set"], chosen["persist"]["team_factor"])
     batch.set_scheduler(G.Batch.PERSISTENT if chosen["scheduler"] == "persistent" else G.Batch.LEVELS)
