@@ -480,48 +480,6 @@ def serve_main(args) -> dict:
             "device": dev_res, "host_staged": host_res}
 
 
-ACCURACY_CASES = [((128, 128, 128), "f64"), ((512, 512, 512), "f32"), ((512, 512, 512), "f64"),
-                  ((768, 768, 768), "f32"), ((1024, 512, 2048), "f32"), ((1024, 1024, 1024), "f32")]
-
-
-def accuracy(cases) -> dict:
-    """Relative L2 error of the GPU R2C against the FP64 oracle (pocketfft
-    on the same, precision-rounded input) and of the normalised round trip,
-    per config grid, at full size."""
-    import numpy as np
-    import torch
-    import gpu_fft_amd as G
-    from oracle import gfft_oracle as O
-    workers = max(1, min(16, os.cpu_count() or 1))
-    out = {}
-    for (nx, ny, nz), prec in cases:
-        log(f"accuracy {nx}x{ny}x{nz} {prec}")
-        rdt = np.float32 if prec == "f32" else np.float64
-        fp = O.pad_real(O.random_field(nx, ny, nz, seed=1)).astype(rdt)
-        buf = torch.from_numpy(fp).to("cuda:0")
-        with G.Plan(nx, ny, nz, precision=prec) as p:
-            p.r2c(buf)
-            torch.cuda.synchronize()
-            got = G.complex_view(buf, nx, ny, nz).cpu().numpy()
-            want = O.r2c_p1(fp, nz, workers=workers)
-            num = den = 0.0
-            for i in range(0, nx, 64):  # chunked norms keep host memory bounded
-                d = got[i:i + 64].astype(np.complex128) - want[i:i + 64]
-                num += float(np.vdot(d, d).real)
-                den += float(np.vdot(want[i:i + 64], want[i:i + 64]).real)
-            del got, want
-            p.set_scale(1.0, 1.0 / (nx * ny * nz))
-            p.c2r(buf)
-            torch.cuda.synchronize()
-            back = buf.cpu().numpy()[..., :nz]
-            rt = O.rel_l2(back, fp[..., :nz])
-        out[f"{nx}x{ny}x{nz}_{prec}"] = {"r2c_rel_l2": float(np.sqrt(num / den)), "round_trip_rel_l2": float(rt),
-                                         "tolerance": 1e-6 if prec == "f32" else 1e-13}
-        del buf, fp, back
-        torch.cuda.empty_cache()
-    return out
-
-
 def copy_calibration(nx, ny, nz, prec, steps):
     """A plain device copy of S bytes, for HBM counter calibration under rocprofv3."""
     import torch
@@ -660,8 +618,6 @@ def main():
                     help="server waiters' sleep between completion queries (0 = blocking, -1 = default 20)")
     ap.add_argument("--serve-device-only", action="store_true", help="serving: device requests only")
     ap.add_argument("--serve-streams", type=int, default=3, help="server streams")
-    ap.add_argument("--accuracy", action="store_true",
-                    help="relative L2 error vs the FP64 oracle at every config grid (no timing)")
     ap.add_argument("--no-extras", action="store_true",
                     help="default run without the batched and served sub-results")
     ap.add_argument("--prec", default="f32", choices=["f32", "f64"])
@@ -690,10 +646,6 @@ def main():
         return
     if args.serve:
         print(json.dumps(serve_main(args)), flush=True)
-        return
-    if args.accuracy:
-        print(json.dumps({"metric": "relative L2 error vs the FP64 oracle per config",
-                          "results": accuracy(ACCURACY_CASES)}), flush=True)
         return
     if args.copy_calibration:
         copy_calibration(nx, ny, nz, args.prec, args.steps)
